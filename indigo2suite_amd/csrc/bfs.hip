@@ -1,0 +1,594 @@
+// BFS on gfx950: unweighted levels from `src`, INT_MAX for unreachable.
+//
+// Reference family: codes/cuda/bfs-cuda (180 variants).  The styles below
+// cover its axes on wave64 hardware:
+//   v_topo_push_*   [C-BVT] BFS_CUDA_V_Topo_Push_..._Thread:46-119 (warp/block
+//                   siblings map to one wave64 / one 256-lane workgroup)
+//   v_topo_pull_*   BFS_CUDA_V_Topo_Pull_..._Thread:56-78
+//   *_determ        level-synchronous rule `if (s == iter)` (Determ siblings :65)
+//   *_cas / *_seqcst  CAS-loop and seq_cst/system-scope atomicMin (the
+//                   reference's Atomic vs CudaAtomic axis)
+//   v_data_push_*   [C-BVD]:46-139 worklist with NonDup stamps
+//   e_topo_push_*   [C-BET]:56-115 one edge per item
+//   e_data_push_*   [C-BED]:46-140 edge worklist, stamps on edge ids
+//   do_hybrid       direction-optimizing BFS (no reference counterpart):
+//                   top-down from a chunked queue, bottom-up over unvisited
+//                   vertices, switched per level on the device
+// All styles produce identical level arrays (BFS levels are unique).
+#include "igx_internal.hpp"
+#include "igx_frontier.hpp"
+
+namespace igx {
+namespace {
+
+enum Gran { kThread = 0, kWaveG = 1, kBlockG = 2 };
+
+__global__ void __launch_bounds__(kBlock) bfs_init(int src, int* __restrict__ dist, int n)
+{
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) dist[v] = (v == src) ? 0 : kInf;
+}
+
+__global__ void __launch_bounds__(kBlock) fill_int(int* __restrict__ p, int value, long long count)
+{
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) p[i] = value;
+}
+
+__global__ void finalize_flag_kernel(Ctl* ctl)
+{
+  if (threadIdx.x == 0 && !round_done(ctl)) finalize_flag_round(ctl);
+}
+
+__global__ void finalize_worklist_kernel(Ctl* ctl)
+{
+  if (threadIdx.x == 0 && !round_done(ctl)) finalize_worklist_round(ctl);
+}
+
+__device__ __forceinline__ void raise_flag(Ctl* ctl, bool updated)
+{
+  const uint64_t any = ballot(updated);
+  if (any != 0 && lane_id() == (__ffsll((unsigned long long)any) - 1)) store_relaxed(&ctl->flag, 1);
+}
+
+// --------------------------------------------------- vertex, topology ----
+template <int G, int AK, bool DETERM, bool PERSIST>
+__global__ void __launch_bounds__(kBlock) bfs_vtopo_push(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int n,
+                                                          int* __restrict__ dist, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  bool updated = false;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int s = dist[v];
+      if (s != kInf && (!DETERM || s == iter)) {
+        const int nd = s + 1;
+        for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+          if (atomic_min_k<AK>(&dist[nlist[i]], nd) > nd) updated = true;
+        }
+      }
+    }
+  } else if constexpr (G == kWaveG) {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int s = dist[v];
+      if (s != kInf && (!DETERM || s == iter)) {
+        const int nd = s + 1;
+        for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) {
+          if (atomic_min_k<AK>(&dist[nlist[i]], nd) > nd) updated = true;
+        }
+      }
+    }
+  } else {
+    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+      const int s = dist[v];
+      if (s != kInf && (!DETERM || s == iter)) {
+        const int nd = s + 1;
+        for (int i = nindex[v] + (int)threadIdx.x; i < nindex[v + 1]; i += kBlock) {
+          if (atomic_min_k<AK>(&dist[nlist[i]], nd) > nd) updated = true;
+        }
+      }
+    }
+  }
+  raise_flag(ctl, updated);
+  if constexpr (PERSIST) finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) bfs_vtopo_pull(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int n,
+                                                          int* __restrict__ dist, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int d0 = dist[v];
+      int d = d0;
+      for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+        const int s = load_relaxed(&dist[nlist[i]]);
+        if (s != kInf && s + 1 < d) d = s + 1;
+      }
+      if (d < d0) {
+        store_relaxed(&dist[v], d);
+        updated = true;
+      }
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int d0 = dist[v];
+      int d = d0;
+      for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) {
+        const int s = load_relaxed(&dist[nlist[i]]);
+        if (s != kInf && s + 1 < d) d = s + 1;
+      }
+      #pragma unroll
+      for (int k = kWave / 2; k > 0; k >>= 1) d = min(d, __shfl_xor(d, k, kWave));
+      if (d < d0) {
+        if (lane == 0) store_relaxed(&dist[v], d);
+        updated = true;
+      }
+    }
+  }
+  raise_flag(ctl, updated);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+// ------------------------------------------------------- edge, topology ----
+template <int AK>
+__global__ void __launch_bounds__(kBlock) bfs_etopo_push(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                          int m, int* __restrict__ dist, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  const int stride = gridDim.x * kBlock;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+    const int s = dist[sp[e]];
+    if (s != kInf) {
+      const int nd = s + 1;
+      const int dst = nlist[e];
+      if (dist[dst] > nd && atomic_min_k<AK>(&dist[dst], nd) > nd) updated = true;
+    }
+  }
+  raise_flag(ctl, updated);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+// ------------------------------------------------------- vertex, data ----
+__global__ void __launch_bounds__(kBlock) bfs_vdata_init(int src, int* __restrict__ wl, Ctl* ctl)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    wl[0] = src;
+    ctl->wl_count[0] = 1;
+    ctl->wl_count[1] = 0;
+  }
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) bfs_vdata_push(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int* __restrict__ dist,
+                                                          int* __restrict__ wl0, int* __restrict__ wl1,
+                                                          int* __restrict__ stamp, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int p = iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int tag = iter + 1;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+      const int v = wl_in[idx];
+      const int s = dist[v];
+      if (s != kInf) {
+        const int nd = s + 1;
+        for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+          const int dst = nlist[i];
+          bool push = false;
+          if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) {
+            push = atomic_max_relaxed(&stamp[dst], tag) != tag;
+          }
+          const int slot = wave_push_slot(out_count, push);
+          if (push) wl_out[slot] = dst;
+        }
+      }
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int idx = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; idx < count; idx += waves) {
+      const int v = wl_in[idx];
+      const int s = dist[v];
+      if (s != kInf) {
+        const int nd = s + 1;
+        const int beg = nindex[v], end = nindex[v + 1];
+        for (int base = beg; base < end; base += kWave) {
+          const int i = base + lane;
+          bool push = false;
+          if (i < end) {
+            const int dst = nlist[i];
+            if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) {
+              push = atomic_max_relaxed(&stamp[dst], tag) != tag;
+            }
+            const int slot = wave_push_slot(out_count, push);
+            if (push) wl_out[slot] = dst;
+          }
+        }
+      }
+    }
+  }
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+// --------------------------------------------------------- edge, data ----
+__global__ void __launch_bounds__(kBlock) bfs_edata_init(int src, const int* __restrict__ nindex,
+                                                          int* __restrict__ wl, Ctl* ctl)
+{
+  if (blockIdx.x != 0) return;
+  const int beg = nindex[src], end = nindex[src + 1];
+  for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) wl[i - beg] = i;
+  if (threadIdx.x == 0) {
+    ctl->wl_count[0] = end - beg;
+    ctl->wl_count[1] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) bfs_edata_push(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                          int* __restrict__ dist, int* __restrict__ wl0,
+                                                          int* __restrict__ wl1, int* __restrict__ stamp, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int p = iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int tag = iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+    const int e = wl_in[idx];
+    const int s = dist[sp[e]];
+    if (s != kInf) {
+      const int nd = s + 1;
+      const int dst = nlist[e];
+      if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) {
+        for (int j = nindex[dst]; j < nindex[dst + 1]; j++) {
+          const bool push = atomic_max_relaxed(&stamp[j], tag) != tag;
+          const int slot = wave_push_slot(out_count, push);
+          if (push) wl_out[slot] = j;
+        }
+      }
+    }
+  }
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+// ------------------------------------------ direction-optimizing hybrid ----
+// Per level one kernel: top-down expansion of the chunk queue, or bottom-up
+// over every vertex.  The last block picks the next direction with Beamer's
+// rule (top-down -> bottom-up when the next frontier's edges exceed the
+// unvisited edges / kAlpha; back when the frontier has < n / kBeta vertices).
+constexpr int kAlpha = 14;
+constexpr int kBeta = 24;
+
+__global__ void __launch_bounds__(kBlock) bfs_do_init(int src, const int* __restrict__ nindex, int m,
+                                                       Entry* __restrict__ q0, Ctl* ctl)
+{
+  if (blockIdx.x != 0) return;
+  const int beg = nindex[src], end = nindex[src + 1];
+  const int c = chunks_of(end - beg);
+  for (int k = threadIdx.x; k < c; k += kBlock) {
+    const int b = beg + k * kChunk;
+    q0[k] = Entry{src, b, (b + kChunk < end) ? b + kChunk : end, 0};
+  }
+  if (threadIdx.x == 0) {
+    ctl->wl_count[0] = c;
+    ctl->wl_count[1] = 0;
+    ctl->mode = 0;
+    ctl->level = 0;
+    ctl->edges_next = 0;
+    ctl->aux_ll[0] = 0;
+    ctl->edges_unvisited = (long long)m - (end - beg);
+  }
+}
+
+__device__ __forceinline__ void block_add_ll(long long* target, long long v, long long* lds)
+{
+  v = wave_sum_ll(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) t += lds[k];
+    if (t != 0) atomicAdd((unsigned long long*)target, (unsigned long long)t);
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ nindex,
+                                                        const int* __restrict__ nlist, int n,
+                                                        int* __restrict__ dist, Entry* __restrict__ q0,
+                                                        Entry* __restrict__ q1, Ctl* ctl)
+{
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ Entry s_ent[kWavesPerBlock][kWave];
+  __shared__ long long s_red[kWavesPerBlock];
+  if (round_done(ctl)) return;
+  const int level = ctl->level;
+  const int mode = ctl->mode;
+  const int p = level & 1;
+  const Entry* q_in = p ? q1 : q0;
+  Entry* q_out = p ? q0 : q1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int nd = level + 1;
+  const int w = threadIdx.x / kWave;
+  const int gw = blockIdx.x * kWavesPerBlock + w;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  long long found_v = 0, found_e = 0;
+  if (mode == 0) {
+    const int qsize = ctl->wl_count[p];
+    wave_expand(q_in, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry&, int e) {
+      const int u = nlist[e];
+      bool disc = false;
+      int ub = 0, ue = 0;
+      if (dist[u] == kInf) {
+        int expected = kInf;
+        disc = __hip_atomic_compare_exchange_strong(&dist[u], &expected, nd, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (disc) {
+          ub = nindex[u];
+          ue = nindex[u + 1];
+          found_v += 1;
+          found_e += ue - ub;
+        }
+      }
+      push_vertex_chunks(q_out, out_count, disc, u, ub, ue);
+    });
+  } else {
+    for (int base = gw * kWave; base < n; base += nwaves * kWave) {
+      const int u = base + lane_id();
+      bool disc = false;
+      int ub = 0, ue = 0;
+      if (u < n && dist[u] == kInf) {
+        ub = nindex[u];
+        ue = nindex[u + 1];
+        for (int i = ub; i < ue; i++) {
+          if (dist[nlist[i]] == level) {
+            disc = true;
+            break;
+          }
+        }
+        if (disc) {
+          dist[u] = nd;
+          found_v += 1;
+          found_e += ue - ub;
+        }
+      }
+      push_vertex_chunks(q_out, out_count, disc, u, ub, ue);
+    }
+  }
+  block_add_ll(&ctl->aux_ll[0], found_v, s_red);
+  block_add_ll(&ctl->edges_next, found_e, s_red);
+  finish_round(ctl, [&] {
+    const long long nf = load_relaxed(&ctl->aux_ll[0]);
+    const long long mf = load_relaxed(&ctl->edges_next);
+    ctl->edges_unvisited -= mf;
+    if (nf == 0) {
+      ctl->done = 1;
+    } else if (ctl->mode == 0) {
+      if (mf > ctl->edges_unvisited / kAlpha) ctl->mode = 1;
+    } else {
+      if (nf < (long long)ctl->n / kBeta) ctl->mode = 0;
+    }
+    ctl->wl_count[p] = 0;
+    ctl->aux_ll[0] = 0;
+    ctl->edges_next = 0;
+    ctl->level += 1;
+    ctl->iter += 1;
+  });
+}
+
+// ------------------------------------------------------------ host side ----
+struct Bufs {
+  int* wl0;
+  int* wl1;
+  int* stamp;
+};
+
+template <typename K>
+int run_topo(const igx_graph* g, int src, int* d_dist, igx_run_info* info, int grid, long long max_rounds,
+             K kernel_launch, bool need_finalize)
+{
+  Runtime& R = rt();
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(bfs_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, d_dist, g->nodes);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        kernel_launch(grid);
+        if (need_finalize) hipLaunchKernelGGL(finalize_flag_kernel, dim3(1), dim3(64), 0, R.stream, R.d_ctl);
+      },
+      max_rounds, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
+template <int G, int AK, bool DETERM, bool PERSIST>
+int style_vtopo_push(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  int grid = R.persist_blocks;
+  if (!PERSIST) {
+    grid = (G == kThread) ? blocks_for(g->nodes, kBlock)
+                          : (G == kWaveG ? blocks_for(g->nodes, kWavesPerBlock) : g->nodes);
+  }
+  return run_topo(g, src, dist, info, grid, (long long)g->nodes + 2,
+                  [&](int gr) {
+                    hipLaunchKernelGGL((bfs_vtopo_push<G, AK, DETERM, PERSIST>), dim3(gr), dim3(kBlock), 0, R.stream,
+                                       g->nindex, g->nlist, g->nodes, dist, R.d_ctl);
+                  },
+                  !PERSIST);
+}
+
+template <int G>
+int style_vtopo_pull(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  return run_topo(g, src, dist, info, R.persist_blocks, (long long)g->nodes + 2,
+                  [&](int gr) {
+                    hipLaunchKernelGGL((bfs_vtopo_pull<G>), dim3(gr), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                                       g->nodes, dist, R.d_ctl);
+                  },
+                  false);
+}
+
+template <int AK>
+int style_etopo_push(const igx_graph* g, int src, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const int* sp = (const int*)aux;
+  return run_topo(g, src, dist, info, R.persist_blocks, (long long)g->nodes + 2,
+                  [&](int gr) {
+                    hipLaunchKernelGGL((bfs_etopo_push<AK>), dim3(gr), dim3(kBlock), 0, R.stream, g->nlist, sp,
+                                       g->edges, dist, R.d_ctl);
+                  },
+                  false);
+}
+
+int get_worklists(size_t items, size_t stamp_items, Bufs* b)
+{
+  void* p = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * items, &p));
+  b->wl0 = (int*)p;
+  IGX_TRY(ws_get(WS_B, sizeof(int) * items, &p));
+  b->wl1 = (int*)p;
+  IGX_TRY(ws_get(WS_C, sizeof(int) * stamp_items, &p));
+  b->stamp = (int*)p;
+  return IGX_OK;
+}
+
+template <int G>
+int style_vdata_push(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const size_t items = (size_t)std::max(g->edges, g->nodes);
+  Bufs b;
+  IGX_TRY(get_worklists(items, (size_t)g->nodes, &b));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(bfs_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  hipLaunchKernelGGL(fill_int, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, b.stamp, 0,
+                     (long long)g->nodes);
+  hipLaunchKernelGGL(bfs_vdata_init, dim3(1), dim3(64), 0, R.stream, src, b.wl0, R.d_ctl);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        hipLaunchKernelGGL((bfs_vdata_push<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
+                           g->nlist, dist, b.wl0, b.wl1, b.stamp, R.d_ctl);
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
+int style_edata_push(const igx_graph* g, int src, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const int* sp = (const int*)aux;
+  const size_t items = (size_t)std::max(g->edges, g->nodes);
+  Bufs b;
+  IGX_TRY(get_worklists(items, items, &b));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(bfs_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  hipLaunchKernelGGL(fill_int, dim3(blocks_for((long long)items, kBlock)), dim3(kBlock), 0, R.stream, b.stamp, 0,
+                     (long long)items);
+  hipLaunchKernelGGL(bfs_edata_init, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, b.wl0, R.d_ctl);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        hipLaunchKernelGGL(bfs_edata_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                           sp, dist, b.wl0, b.wl1, b.stamp, R.d_ctl);
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
+int style_do_hybrid(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const size_t cap = (size_t)g->nodes + (size_t)g->edges / kChunk + 64;
+  void* p0 = nullptr;
+  void* p1 = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &p0));
+  IGX_TRY(ws_get(WS_E, sizeof(Entry) * cap, &p1));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(bfs_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  hipLaunchKernelGGL(bfs_do_init, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, g->edges, (Entry*)p0, R.d_ctl);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        hipLaunchKernelGGL(bfs_do_level, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                           g->nodes, dist, (Entry*)p0, (Entry*)p1, R.d_ctl);
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
+const StyleEntry kStyles[] = {
+    {"v_topo_push_thread", style_vtopo_push<kThread, kAtomicRelaxed, false, true>, 0},
+    {"v_topo_push_wave", style_vtopo_push<kWaveG, kAtomicRelaxed, false, true>, 0},
+    {"v_topo_push_block", style_vtopo_push<kBlockG, kAtomicRelaxed, false, true>, 0},
+    {"v_topo_push_thread_nonpersist", style_vtopo_push<kThread, kAtomicRelaxed, false, false>, 0},
+    {"v_topo_push_thread_determ", style_vtopo_push<kThread, kAtomicRelaxed, true, true>, 0},
+    {"v_topo_push_thread_cas", style_vtopo_push<kThread, kAtomicCas, false, true>, 0},
+    {"v_topo_push_thread_seqcst", style_vtopo_push<kThread, kAtomicSeqCst, false, true>, 0},
+    {"v_topo_pull_thread", style_vtopo_pull<kThread>, 0},
+    {"v_topo_pull_wave", style_vtopo_pull<kWaveG>, 0},
+    {"v_data_push_thread", style_vdata_push<kThread>, 0},
+    {"v_data_push_wave", style_vdata_push<kWaveG>, 0},
+    {"e_topo_push_thread", style_etopo_push<kAtomicRelaxed>, 1},
+    {"e_data_push_thread", style_edata_push, 1},
+    {"do_hybrid", style_do_hybrid, 0},
+};
+
+}  // namespace
+
+const StyleEntry* bfs_styles(int* count)
+{
+  *count = (int)(sizeof(kStyles) / sizeof(kStyles[0]));
+  return kStyles;
+}
+
+}  // namespace igx

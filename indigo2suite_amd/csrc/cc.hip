@@ -1,0 +1,456 @@
+// Connected components on gfx950: label[v] = minimum vertex id of v's
+// component (the fixed point of the reference's min-label propagation,
+// `label[v] = v` then repeated atomicMin along edges).  Inputs are undirected
+// (both directions stored), as all Indigo2 inputs are.
+//
+// Reference family: codes/cuda/cc-cuda (168 variants).
+//   v_topo_push_*   [C-CVT] CC_CUDA_V_Topo_Push_..._Thread:47-116
+//   v_topo_pull_*   CC_CUDA_V_Topo_Pull siblings (read neighbours, write self)
+//   e_topo_push_*   [C-CET]:57-69 one edge per item
+//   v_data_push_*   [C-CVD]:46-136 worklist seeded with every vertex
+//   e_data_push_*   [C-CED]:48-80 edge worklist seeded with every edge
+//   uf_hook_compress  union-find (no reference counterpart): parents start at
+//                   a smaller neighbour, every edge (v, u<v) hooks the larger
+//                   root under the smaller with a CAS, and a final pass points
+//                   every vertex at its root.  Roots only ever link to smaller
+//                   ids, so each root is its component's minimum id.
+#include "igx_internal.hpp"
+#include "igx_frontier.hpp"
+
+namespace igx {
+namespace {
+
+enum Gran { kThread = 0, kWaveG = 1, kBlockG = 2 };
+
+__global__ void __launch_bounds__(kBlock) cc_init(int* __restrict__ label, int n)
+{
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) label[v] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) cc_fill(int* __restrict__ p, int value, long long count)
+{
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) p[i] = value;
+}
+
+__global__ void cc_finalize_flag_kernel(Ctl* ctl)
+{
+  if (threadIdx.x == 0 && !round_done(ctl)) finalize_flag_round(ctl);
+}
+
+__device__ __forceinline__ void cc_raise_flag(Ctl* ctl, bool updated)
+{
+  const uint64_t any = ballot(updated);
+  if (any != 0 && lane_id() == (__ffsll((unsigned long long)any) - 1)) store_relaxed(&ctl->flag, 1);
+}
+
+template <int G, int AK, bool PERSIST>
+__global__ void __launch_bounds__(kBlock) cc_vtopo_push(const int* __restrict__ nindex,
+                                                         const int* __restrict__ nlist, int n,
+                                                         int* __restrict__ label, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int lv = label[v];
+      for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+        const int dst = nlist[i];
+        if (label[dst] > lv && atomic_min_k<AK>(&label[dst], lv) > lv) updated = true;
+      }
+    }
+  } else if constexpr (G == kWaveG) {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int lv = label[v];
+      for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) {
+        const int dst = nlist[i];
+        if (label[dst] > lv && atomic_min_k<AK>(&label[dst], lv) > lv) updated = true;
+      }
+    }
+  } else {
+    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+      const int lv = label[v];
+      for (int i = nindex[v] + (int)threadIdx.x; i < nindex[v + 1]; i += kBlock) {
+        const int dst = nlist[i];
+        if (label[dst] > lv && atomic_min_k<AK>(&label[dst], lv) > lv) updated = true;
+      }
+    }
+  }
+  cc_raise_flag(ctl, updated);
+  if constexpr (PERSIST) finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+__global__ void __launch_bounds__(kBlock) cc_vtopo_pull(const int* __restrict__ nindex,
+                                                         const int* __restrict__ nlist, int n,
+                                                         int* __restrict__ label, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const int l0 = label[v];
+    int l = l0;
+    for (int i = nindex[v]; i < nindex[v + 1]; i++) l = min(l, load_relaxed(&label[nlist[i]]));
+    if (l < l0) {
+      store_relaxed(&label[v], l);
+      updated = true;
+    }
+  }
+  cc_raise_flag(ctl, updated);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+__global__ void __launch_bounds__(kBlock) cc_etopo_push(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                         int m, int* __restrict__ label, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  const int stride = gridDim.x * kBlock;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+    const int lv = label[sp[e]];
+    const int dst = nlist[e];
+    if (label[dst] > lv && atomic_min_relaxed(&label[dst], lv) > lv) updated = true;
+  }
+  cc_raise_flag(ctl, updated);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+// worklist seeded with all items [0, count)
+__global__ void __launch_bounds__(kBlock) cc_seed(int* __restrict__ wl, int count, Ctl* ctl)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < count) wl[i] = i;
+  if (i == 0) {
+    ctl->wl_count[0] = count;
+    ctl->wl_count[1] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) cc_vdata_push(const int* __restrict__ nindex,
+                                                         const int* __restrict__ nlist, int* __restrict__ label,
+                                                         int* __restrict__ wl0, int* __restrict__ wl1,
+                                                         int* __restrict__ stamp, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int p = iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int tag = iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+    const int v = wl_in[idx];
+    const int lv = label[v];
+    for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+      const int dst = nlist[i];
+      bool push = false;
+      if (label[dst] > lv && atomic_min_relaxed(&label[dst], lv) > lv) {
+        push = atomic_max_relaxed(&stamp[dst], tag) != tag;
+      }
+      const int slot = wave_push_slot(out_count, push);
+      if (push) wl_out[slot] = dst;
+    }
+  }
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+__global__ void __launch_bounds__(kBlock) cc_edata_push(const int* __restrict__ nindex,
+                                                         const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                         int* __restrict__ label, int* __restrict__ wl0,
+                                                         int* __restrict__ wl1, int* __restrict__ stamp, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int p = iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int tag = iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+    const int e = wl_in[idx];
+    const int lv = label[sp[e]];
+    const int dst = nlist[e];
+    if (label[dst] > lv && atomic_min_relaxed(&label[dst], lv) > lv) {
+      for (int j = nindex[dst]; j < nindex[dst + 1]; j++) {
+        const bool push = atomic_max_relaxed(&stamp[j], tag) != tag;
+        const int slot = wave_push_slot(out_count, push);
+        if (push) wl_out[slot] = j;
+      }
+    }
+  }
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+// ------------------------------------------------------------- union-find --
+// Parent pointers satisfy parent[x] <= x; roots point at themselves.  Path
+// halving only rewrites non-roots, and roots change only through the CAS in
+// uf_unite, so concurrent finds and unions keep the forest valid.
+__device__ __forceinline__ int uf_find(int* __restrict__ parent, int x)
+{
+  while (true) {
+    const int p = parent[x];
+    if (p == x) return x;
+    const int gp = parent[p];
+    if (gp == p) return p;
+    parent[x] = gp;
+    x = gp;
+  }
+}
+
+__device__ __forceinline__ void uf_unite(int* __restrict__ parent, int a, int b)
+{
+  int ra = uf_find(parent, a);
+  int rb = uf_find(parent, b);
+  while (ra != rb) {
+    const int hi = max(ra, rb), lo = min(ra, rb);
+    int expected = hi;
+    if (__hip_atomic_compare_exchange_strong(&parent[hi], &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)) {
+      return;
+    }
+    // hi was linked meanwhile: `expected` holds its current parent
+    ra = uf_find(parent, expected);
+    rb = uf_find(parent, lo);
+  }
+}
+
+// parent[v] = first smaller neighbour (or v); vertices with more than kChunk
+// edges also get their edge chunks queued for the balanced hook pass.
+__global__ void __launch_bounds__(kBlock) uf_init(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                   int n, int* __restrict__ parent, Entry* __restrict__ bigq,
+                                                   int* __restrict__ bigq_count)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < n; base += stride) {
+    const int v = base + threadIdx.x;
+    int beg = 0, end = 0;
+    if (v < n) {
+      beg = nindex[v];
+      end = nindex[v + 1];
+      int pv = v;
+      for (int i = beg; i < end; i++) {
+        const int u = nlist[i];
+        if (u < v) {
+          pv = u;
+          break;
+        }
+      }
+      parent[v] = pv;
+    }
+    push_vertex_chunks(bigq, bigq_count, (end - beg) > kChunk, v, beg, end);
+  }
+}
+
+// Hook every edge (v, u) with u < v.  Rows of at most kChunk edges are walked
+// 64 rows per wave over their contiguous edge range (coalesced nlist reads);
+// larger rows come from the chunk queue so hub edges spread over all waves.
+__global__ void __launch_bounds__(kBlock) uf_hook(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                   int n, int* __restrict__ parent, const Entry* __restrict__ bigq,
+                                                   const int* __restrict__ bigq_count)
+{
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
+  __shared__ Entry s_ent[kWavesPerBlock][kWave];
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int gw = blockIdx.x * kWavesPerBlock + w;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  int* excl = s_excl[w];
+  int* rbeg = s_beg[w];
+  for (int v0 = gw * kWave; v0 < n; v0 += nwaves * kWave) {
+    const int v = v0 + lane;
+    int beg = 0, len = 0;
+    if (v < n) {
+      beg = nindex[v];
+      len = nindex[v + 1] - beg;
+      if (len > kChunk) len = 0;
+    }
+    const int incl = wave_incl_scan(len);
+    const int total = wave_bcast(incl, kWave - 1);
+    excl[lane] = incl - len;
+    rbeg[lane] = beg;
+    __builtin_amdgcn_wave_barrier();
+    for (int tb = 0; tb < total; tb += kWave) {
+      const int t = tb + lane;
+      if (t < total) {
+        int lo = 0, hi = kWave;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (excl[mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        const int row = v0 + lo;
+        const int e = rbeg[lo] + (t - excl[lo]);
+        const int u = nlist[e];
+        if (u < row) uf_unite(parent, row, u);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const int qsize = *bigq_count;
+  wave_expand(bigq, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
+    const int u = nlist[e];
+    if (u < en.v) uf_unite(parent, en.v, u);
+  });
+}
+
+__global__ void __launch_bounds__(kBlock) uf_compress(int n, int* __restrict__ parent)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) parent[v] = uf_find(parent, v);
+}
+
+// ------------------------------------------------------------ host side ----
+template <typename K>
+int run_cc_topo(const igx_graph* g, int* d_label, igx_run_info* info, K launch, bool need_finalize)
+{
+  Runtime& R = rt();
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(cc_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, d_label, g->nodes);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        launch();
+        if (need_finalize) hipLaunchKernelGGL(cc_finalize_flag_kernel, dim3(1), dim3(64), 0, R.stream, R.d_ctl);
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
+template <int G, int AK, bool PERSIST>
+int style_vtopo_push(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* label = (int*)out;
+  int grid = R.persist_blocks;
+  if (!PERSIST) {
+    grid = (G == kThread) ? blocks_for(g->nodes, kBlock)
+                          : (G == kWaveG ? blocks_for(g->nodes, kWavesPerBlock) : g->nodes);
+  }
+  return run_cc_topo(g, label, info, [&] {
+    hipLaunchKernelGGL((cc_vtopo_push<G, AK, PERSIST>), dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->nodes, label, R.d_ctl);
+  }, !PERSIST);
+}
+
+int style_vtopo_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* label = (int*)out;
+  return run_cc_topo(g, label, info, [&] {
+    hipLaunchKernelGGL(cc_vtopo_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->nodes, label, R.d_ctl);
+  }, false);
+}
+
+int style_etopo_push(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* label = (int*)out;
+  const int* sp = (const int*)aux;
+  return run_cc_topo(g, label, info, [&] {
+    hipLaunchKernelGGL(cc_etopo_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, sp, g->edges,
+                       label, R.d_ctl);
+  }, false);
+}
+
+template <bool EDGE>
+int style_data_push(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* label = (int*)out;
+  const int* sp = (const int*)aux;
+  const size_t items = (size_t)std::max(g->edges, g->nodes);
+  const int seed_count = EDGE ? g->edges : g->nodes;
+  const size_t stamp_items = EDGE ? items : (size_t)g->nodes;
+  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * items, &p0));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * items, &p1));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * stamp_items, &p2));
+  int* wl0 = (int*)p0;
+  int* wl1 = (int*)p1;
+  int* stamp = (int*)p2;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(cc_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, label, g->nodes);
+  hipLaunchKernelGGL(cc_fill, dim3(blocks_for((long long)stamp_items, kBlock)), dim3(kBlock), 0, R.stream, stamp, 0,
+                     (long long)stamp_items);
+  hipLaunchKernelGGL(cc_seed, dim3(blocks_for(seed_count, kBlock)), dim3(kBlock), 0, R.stream, wl0, seed_count,
+                     R.d_ctl);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        if (EDGE) {
+          hipLaunchKernelGGL(cc_edata_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                             sp, label, wl0, wl1, stamp, R.d_ctl);
+        } else {
+          hipLaunchKernelGGL(cc_vdata_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                             label, wl0, wl1, stamp, R.d_ctl);
+        }
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
+int style_uf(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* parent = (int*)out;
+  // rows with more than kChunk edges need ceil(deg / kChunk) < 2 * deg / kChunk slots
+  const size_t cap = 2 * ((size_t)g->edges / kChunk) + 64;
+  void *pq = nullptr, *pc = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &pq));
+  IGX_TRY(ws_get(WS_E, sizeof(int) * 64, &pc));
+  const int grid = R.persist_blocks;
+  IGX_TRY(timer_start());
+  hipLaunchKernelGGL(cc_fill, dim3(1), dim3(kBlock), 0, R.stream, (int*)pc, 0, 1LL);
+  hipLaunchKernelGGL(uf_init, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent,
+                     (Entry*)pq, (int*)pc);
+  hipLaunchKernelGGL(uf_hook, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent,
+                     (const Entry*)pq, (const int*)pc);
+  hipLaunchKernelGGL(uf_compress, dim3(grid), dim3(kBlock), 0, R.stream, g->nodes, parent);
+  IGX_HIP(hipGetLastError());
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = 1;
+  info->launches = 4;
+  return IGX_OK;
+}
+
+const StyleEntry kStyles[] = {
+    {"v_topo_push_thread", style_vtopo_push<kThread, kAtomicRelaxed, true>, 0},
+    {"v_topo_push_wave", style_vtopo_push<kWaveG, kAtomicRelaxed, true>, 0},
+    {"v_topo_push_block", style_vtopo_push<kBlockG, kAtomicRelaxed, true>, 0},
+    {"v_topo_push_thread_nonpersist", style_vtopo_push<kThread, kAtomicRelaxed, false>, 0},
+    {"v_topo_push_thread_cas", style_vtopo_push<kThread, kAtomicCas, true>, 0},
+    {"v_topo_push_thread_seqcst", style_vtopo_push<kThread, kAtomicSeqCst, true>, 0},
+    {"v_topo_pull_thread", style_vtopo_pull, 0},
+    {"e_topo_push_thread", style_etopo_push, 1},
+    {"v_data_push_thread", style_data_push<false>, 0},
+    {"e_data_push_thread", style_data_push<true>, 1},
+    {"uf_hook_compress", style_uf, 0},
+};
+
+}  // namespace
+
+const StyleEntry* cc_styles(int* count)
+{
+  *count = (int)(sizeof(kStyles) / sizeof(kStyles[0]));
+  return kStyles;
+}
+
+}  // namespace igx

@@ -1,0 +1,130 @@
+// Host-side internals of libindigo_hip.so: runtime state, error handling,
+// cached device workspaces, the round loop and the style registry types.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "igx_graph.h"
+#include "indigo_hip.h"
+#include "igx_device.hpp"
+
+#define IGX_HIP(call)                                  \
+  do {                                                 \
+    hipError_t igx_e_ = (call);                        \
+    if (igx_e_ != hipSuccess) {                        \
+      igx::set_last_hip_error(igx_e_, #call);          \
+      return IGX_ERR_HIP;                              \
+    }                                                  \
+  } while (0)
+
+#define IGX_TRY(call)                \
+  do {                               \
+    int igx_s_ = (call);             \
+    if (igx_s_ != IGX_OK) return igx_s_; \
+  } while (0)
+
+namespace igx {
+
+void set_last_hip_error(hipError_t e, const char* what);
+
+struct Runtime {
+  bool ready = false;
+  int device = 0;
+  int cus = 0;                 // compute units
+  int max_threads_per_cu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  Ctl* d_ctl = nullptr;        // device control block
+  Ctl* h_ctl = nullptr;        // pinned host mirror
+  int persist_blocks = 0;      // grid of persistent kernels
+};
+
+Runtime& rt();
+int ensure_init();
+
+// Cached device buffers, one per slot, grown on demand and kept until
+// igx_release_workspace() -- allocation stays outside the timed region.
+enum WsSlot {
+  WS_A = 0, WS_B, WS_C, WS_D, WS_E, WS_F, WS_G, WS_H, WS_SP, WS_SCRATCH, WS_COUNT
+};
+int ws_get(int slot, size_t bytes, void** ptr);
+
+// Enqueue round kernels in growing batches until the control block reports
+// done or max_rounds were enqueued.  `launch(r)` enqueues round r.
+// Returns IGX_OK, IGX_ERR_LIMIT when max_rounds did not converge, or an error.
+template <typename F>
+int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches)
+{
+  Runtime& R = rt();
+  long long launched = 0;
+  int batch = 2;
+  while (true) {
+    const long long k = (launched + batch > max_rounds) ? (max_rounds - launched) : batch;
+    for (long long i = 0; i < k; i++) {
+      launch(launched + i);
+    }
+    IGX_HIP(hipGetLastError());
+    launched += k;
+    IGX_HIP(hipMemcpyAsync(R.h_ctl, R.d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, R.stream));
+    IGX_HIP(hipStreamSynchronize(R.stream));
+    if (R.h_ctl->done) break;
+    if (launched >= max_rounds) {
+      if (rounds_done) *rounds_done = R.h_ctl->iter;
+      if (launches) *launches = (int)launched;
+      return IGX_ERR_LIMIT;
+    }
+    batch = (batch < 64) ? batch * 2 : 64;
+  }
+  if (rounds_done) *rounds_done = R.h_ctl->iter;
+  if (launches) *launches = (int)launched;
+  return IGX_OK;
+}
+
+// Clear the control block on the stream (kernel, not a memset node).
+int ctl_reset(int n);
+
+inline int blocks_for(long long items, int per_block)
+{
+  long long b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > 0x7fffffffLL) b = 0x7fffffffLL;
+  return (int)b;
+}
+
+// ---------------------------------------------------------------- styles --
+typedef int (*RunFn)(const igx_graph* g, int src, void* d_out, const void* d_aux, igx_run_info* info);
+
+struct StyleEntry {
+  const char* name;
+  RunFn fn;
+  int edge_based;  // needs or benefits from sp[]
+};
+
+const StyleEntry* styles_of(int algo, int* count);
+
+// per-algorithm registries (defined in each algorithm's translation unit)
+const StyleEntry* bfs_styles(int* count);
+const StyleEntry* sssp_styles(int* count);
+const StyleEntry* cc_styles(int* count);
+const StyleEntry* mis_styles(int* count);
+const StyleEntry* pr_styles(int* count);
+const StyleEntry* tc_styles(int* count);
+
+// PR parameters (eps on the L1 residual, iteration cap)
+struct PrParams {
+  double eps = 1e-4;   // pr_cuda.h:42 (EPSILON)
+  int max_iter = 100;  // pr_cuda.h:45 (MAX_ITER)
+};
+PrParams& pr_params();
+
+// device sp[] of the graph, built on the device and cached per graph pointer
+int device_sp(const igx_graph* g, const int** d_sp);
+
+// events bracketing the timed region
+int timer_start();
+int timer_stop(double* seconds);
+
+}  // namespace igx

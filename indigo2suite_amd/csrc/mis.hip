@@ -1,0 +1,308 @@
+// Maximal independent set on gfx950.  Priorities are the reference's hash of
+// (v + 712313887) compared as UNSIGNED with ties broken toward the larger id
+// (codes/omp MIS init and mis_vertex_omp.h:127-132; the CUDA variants compare
+// them as signed int and produce a different set).  With fixed priorities
+// every style converges to the same set: the greedy MIS in descending
+// (priority, id) order.  Output: int32 status, 1 = in the set, 2 = not.
+// Self loops are ignored (the reference never terminates on them).
+//
+// Reference family: codes/cuda/mis-cuda (112 variants), codes/omp/mis-omp.
+//   v_topo_push_*  [C-MVT] MIS_CUDA_V_Topo_Push_..._Thread:45-128 / [O-MIS]
+//   v_topo_pull_*  V_Topo_Pull siblings (exclude self when a neighbour is in)
+//   v_data_pull_*  V_Data_Pull siblings: worklist of undecided vertices
+//   e_topo_thread  [C-MET]:57-127 / [O-MISe]:52-94 edge pass + vertex pass
+#include "igx_internal.hpp"
+
+namespace igx {
+namespace {
+
+constexpr int kUndecided = 0;
+constexpr int kIncluded = 1;
+constexpr int kExcluded = 2;
+
+// u outranks v under the unsigned-priority order with id tie-break
+__device__ __forceinline__ bool outranks(uint32_t pu, int u, uint32_t pv, int v)
+{
+  return (pu > pv) || (pu == pv && u > v);
+}
+
+__global__ void __launch_bounds__(kBlock) mis_init(int* __restrict__ status, int n)
+{
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) status[v] = kUndecided;
+}
+
+__device__ __forceinline__ void mis_raise_flag(Ctl* ctl, bool pending)
+{
+  const uint64_t any = ballot(pending);
+  if (any != 0 && lane_id() == (__ffsll((unsigned long long)any) - 1)) store_relaxed(&ctl->flag, 1);
+}
+
+// [C-MVT] mis():56-80 with unsigned priorities
+template <bool WAVE>
+__global__ void __launch_bounds__(kBlock) mis_vtopo_push(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int n,
+                                                          int* __restrict__ status, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool pending = false;
+  if constexpr (!WAVE) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      if (load_relaxed(&status[v]) != kUndecided) continue;
+      const uint32_t pv = mis_priority(v);
+      const int beg = nindex[v], end = nindex[v + 1];
+      int i = beg;
+      while (i < end) {
+        const int u = nlist[i];
+        if (u != v && load_relaxed(&status[u]) != kExcluded && outranks(mis_priority(u), u, pv, v)) break;
+        i++;
+      }
+      if (i < end) {
+        pending = true;
+      } else {
+        store_relaxed(&status[v], kIncluded);
+        for (int j = beg; j < end; j++) {
+          const int u = nlist[j];
+          if (u != v) store_relaxed(&status[u], kExcluded);
+        }
+      }
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      if (load_relaxed(&status[v]) != kUndecided) continue;
+      const uint32_t pv = mis_priority(v);
+      const int beg = nindex[v], end = nindex[v + 1];
+      bool blocked = false;
+      for (int i = beg + lane; i < end; i += kWave) {
+        const int u = nlist[i];
+        if (u != v && load_relaxed(&status[u]) != kExcluded && outranks(mis_priority(u), u, pv, v)) blocked = true;
+      }
+      if (ballot(blocked) != 0) {
+        if (lane == 0) pending = true;
+      } else {
+        if (lane == 0) store_relaxed(&status[v], kIncluded);
+        for (int j = beg + lane; j < end; j += kWave) {
+          const int u = nlist[j];
+          if (u != v) store_relaxed(&status[u], kExcluded);
+        }
+      }
+    }
+  }
+  mis_raise_flag(ctl, pending);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+// pull: a vertex excludes itself when a neighbour is in the set and includes
+// itself when every outranking neighbour is excluded
+__device__ __forceinline__ int mis_pull_decide(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                               const int* __restrict__ status, int v)
+{
+  const uint32_t pv = mis_priority(v);
+  bool best = true;
+  for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+    const int u = nlist[i];
+    if (u == v) continue;
+    const int su = load_relaxed(&status[u]);
+    if (su == kIncluded) return kExcluded;
+    if (su != kExcluded && outranks(mis_priority(u), u, pv, v)) best = false;
+  }
+  return best ? kIncluded : kUndecided;
+}
+
+__global__ void __launch_bounds__(kBlock) mis_vtopo_pull(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int n,
+                                                          int* __restrict__ status, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool pending = false;
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    if (load_relaxed(&status[v]) != kUndecided) continue;
+    const int d = mis_pull_decide(nindex, nlist, status, v);
+    if (d == kUndecided) pending = true;
+    else store_relaxed(&status[v], d);
+  }
+  mis_raise_flag(ctl, pending);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+__global__ void __launch_bounds__(kBlock) mis_seed(int* __restrict__ wl, int n, Ctl* ctl)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) wl[i] = i;
+  if (i == 0) {
+    ctl->wl_count[0] = n;
+    ctl->wl_count[1] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) mis_vdata_pull(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int* __restrict__ status,
+                                                          int* __restrict__ wl0, int* __restrict__ wl1, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int p = ctl->iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < count; base += stride) {
+    const int idx = base + threadIdx.x;
+    bool keep = false;
+    int v = 0;
+    if (idx < count) {
+      v = wl_in[idx];
+      if (load_relaxed(&status[v]) == kUndecided) {
+        const int d = mis_pull_decide(nindex, nlist, status, v);
+        if (d == kUndecided) keep = true;
+        else store_relaxed(&status[v], d);
+      }
+    }
+    const int slot = wave_push_slot(out_count, keep);
+    if (keep) wl_out[slot] = v;
+  }
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+// [O-MISe]:52-77 edge pass: exclude the neighbours of included vertices and
+// mark the lower-ranked endpoint of an undecided pair as lost
+__global__ void __launch_bounds__(kBlock) mis_edge_pass(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                         int m, int* __restrict__ status, int* __restrict__ lost,
+                                                         Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int stride = gridDim.x * kBlock;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+    const int src = sp[e];
+    const int dst = nlist[e];
+    if (src == dst) continue;
+    const int ss = load_relaxed(&status[src]);
+    const int sd = load_relaxed(&status[dst]);
+    if (ss == kIncluded) {
+      store_relaxed(&status[dst], kExcluded);
+    } else if (sd == kIncluded) {
+      store_relaxed(&status[src], kExcluded);
+    } else if (ss != kExcluded && sd != kExcluded) {
+      const bool dst_wins = outranks(mis_priority(dst), dst, mis_priority(src), src);
+      store_relaxed(&lost[dst_wins ? src : dst], 1);
+    }
+  }
+}
+
+// [O-MISe]:79-94 vertex pass: include every undecided vertex that did not lose
+__global__ void __launch_bounds__(kBlock) mis_vertex_pass(int n, int* __restrict__ status, int* __restrict__ lost,
+                                                           Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool pending = false;
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    if (lost[v] == 0) {
+      if (status[v] == kUndecided) status[v] = kIncluded;
+    } else {
+      pending = true;
+      lost[v] = 0;
+    }
+  }
+  mis_raise_flag(ctl, pending);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+// ------------------------------------------------------------ host side ----
+template <typename K>
+int mis_rounds(const igx_graph* g, igx_run_info* info, K launch)
+{
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds([&](long long) { launch(); }, (long long)g->nodes + 2, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
+template <bool WAVE>
+int style_vtopo_push(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* status = (int*)out;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, status, g->nodes);
+  return mis_rounds(g, info, [&] {
+    hipLaunchKernelGGL((mis_vtopo_push<WAVE>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
+                       g->nlist, g->nodes, status, R.d_ctl);
+  });
+}
+
+int style_vtopo_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* status = (int*)out;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, status, g->nodes);
+  return mis_rounds(g, info, [&] {
+    hipLaunchKernelGGL(mis_vtopo_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->nodes, status, R.d_ctl);
+  });
+}
+
+int style_vdata_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* status = (int*)out;
+  void *p0 = nullptr, *p1 = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)g->nodes, &p0));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * (size_t)g->nodes, &p1));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, status, g->nodes);
+  hipLaunchKernelGGL(mis_seed, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, (int*)p0, g->nodes,
+                     R.d_ctl);
+  return mis_rounds(g, info, [&] {
+    hipLaunchKernelGGL(mis_vdata_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       status, (int*)p0, (int*)p1, R.d_ctl);
+  });
+}
+
+int style_etopo(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* status = (int*)out;
+  const int* sp = (const int*)aux;
+  void* pl = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)g->nodes, &pl));
+  int* lost = (int*)pl;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, status, g->nodes);
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, lost, g->nodes);
+  return mis_rounds(g, info, [&] {
+    hipLaunchKernelGGL(mis_edge_pass, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, sp, g->edges,
+                       status, lost, R.d_ctl);
+    hipLaunchKernelGGL(mis_vertex_pass, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nodes, status, lost,
+                       R.d_ctl);
+  });
+}
+
+const StyleEntry kStyles[] = {
+    {"v_topo_push_thread", style_vtopo_push<false>, 0},
+    {"v_topo_push_wave", style_vtopo_push<true>, 0},
+    {"v_topo_pull_thread", style_vtopo_pull, 0},
+    {"v_data_pull_thread", style_vdata_pull, 0},
+    {"e_topo_thread", style_etopo, 1},
+};
+
+}  // namespace
+
+const StyleEntry* mis_styles(int* count)
+{
+  *count = (int)(sizeof(kStyles) / sizeof(kStyles[0]));
+  return kStyles;
+}
+
+}  // namespace igx

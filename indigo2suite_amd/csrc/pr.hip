@@ -1,0 +1,372 @@
+// PageRank (pull, Jacobi) on gfx950, following pr_cuda.h / [C-PRt]:
+//   s0 = 1/n;  s_v <- (1 - d)/n + d * sum_{u in N(v)} s_u / deg(u),  d = 0.85
+// with no dangling-mass redistribution, stopping when the L1 change of an
+// iteration falls below eps or after max_iter iterations.
+//
+// Precision: scores and contributions are FP32; every neighbour sum is
+// compensated (TwoSum per addition; partial sums are merged as (sum, error)
+// pairs), and the per-iteration L1 residual that decides convergence is
+// accumulated in FP64.  Results are checked against an FP64 host oracle.
+//
+// Reference family: codes/cuda/pr-cuda (54 variants).
+//   pull_thread_kahan   [C-PRt] contrib:41-47 + pull:49-70 (thread per vertex)
+//   pull_wave_kahan     PR_CUDA_Pull_..._Warp siblings (one wave64 per vertex)
+//   pull_block_kahan    PR_CUDA_Pull_..._Block siblings (one workgroup per vertex)
+//   pull_balanced_kahan (no reference counterpart) rows classed by degree:
+//                       thread per small row, wave per medium row, and
+//                       large rows split into chunks whose compensated partial
+//                       sums are merged in chunk order
+// The contribution of the next iteration (score / degree) is written by the
+// same kernel that produces the score, so one pass per iteration remains.
+#include "igx_internal.hpp"
+#include "igx_frontier.hpp"
+
+namespace igx {
+namespace {
+
+constexpr float kDamp = 0.85f;  // pr_cuda.h:43
+constexpr int kSmallRow = 32;
+constexpr int kMediumRow = 2048;
+
+struct Pair {
+  float s;  // running sum
+  float e;  // accumulated rounding error (true sum ~ s + e)
+};
+
+__device__ __forceinline__ void two_sum_add(Pair& p, float x)
+{
+  const float t = p.s + x;
+  const float bp = t - p.s;
+  const float err = (p.s - (t - bp)) + (x - bp);
+  p.s = t;
+  p.e += err;
+}
+
+__device__ __forceinline__ Pair pair_merge(Pair a, Pair b)
+{
+  Pair r;
+  r.s = a.s + b.s;
+  const float bp = r.s - a.s;
+  const float err = (a.s - (r.s - bp)) + (b.s - bp);
+  r.e = a.e + b.e + err;
+  return r;
+}
+
+__device__ __forceinline__ Pair wave_merge(Pair p)
+{
+  #pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    Pair o;
+    o.s = __shfl_xor(p.s, d, kWave);
+    o.e = __shfl_xor(p.e, d, kWave);
+    // merge in a lane-order-independent way: lower half first
+    p = ((lane_id() & d) == 0) ? pair_merge(p, o) : pair_merge(o, p);
+  }
+  return p;
+}
+
+__global__ void __launch_bounds__(kBlock) pr_init(const int* __restrict__ nindex, int n, float* __restrict__ score,
+                                                   float* __restrict__ contrib)
+{
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) {
+    const float s0 = 1.0f / (float)n;  // pr_cuda.h:133
+    score[v] = s0;
+    contrib[v] = s0 / (float)(nindex[v + 1] - nindex[v]);
+  }
+}
+
+// finish one vertex: new score, next contribution, FP64 residual term
+__device__ __forceinline__ double pr_finish(int v, Pair acc, float base, const int* __restrict__ nindex,
+                                            float* __restrict__ score, float* __restrict__ contrib_next)
+{
+  const float sum = acc.s + acc.e;
+  const float value = base + kDamp * sum;
+  const float old = score[v];
+  score[v] = value;
+  contrib_next[v] = value / (float)(nindex[v + 1] - nindex[v]);
+  return fabs((double)value - (double)old);
+}
+
+__device__ __forceinline__ void block_add_d(double* target, double v, double* lds)
+{
+  v = wave_sum_d(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) t += lds[k];
+    if (t != 0.0) atomicAdd(target, t);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void pr_finalize_round(Ctl* ctl, double eps, int max_iter)
+{
+  const double r = load_relaxed(&ctl->res);
+  ctl->res_last = r;
+  ctl->res = 0.0;
+  ctl->iter += 1;
+  if (r < eps || ctl->iter >= max_iter) ctl->done = 1;
+}
+
+enum Gran { kThread = 0, kWaveG = 1, kBlockG = 2 };
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) pr_pull(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                   int n, float base, float* __restrict__ score,
+                                                   const float* __restrict__ contrib_cur,
+                                                   float* __restrict__ contrib_next, Ctl* ctl, double eps,
+                                                   int max_iter)
+{
+  __shared__ double s_red[kWavesPerBlock];
+  __shared__ Pair s_pair[kWavesPerBlock];
+  if (round_done(ctl)) return;
+  double err = 0.0;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      Pair acc{0.0f, 0.0f};
+      for (int i = nindex[v]; i < nindex[v + 1]; i++) two_sum_add(acc, contrib_cur[nlist[i]]);
+      err += pr_finish(v, acc, base, nindex, score, contrib_next);
+    }
+  } else if constexpr (G == kWaveG) {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      Pair acc{0.0f, 0.0f};
+      for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) two_sum_add(acc, contrib_cur[nlist[i]]);
+      acc = wave_merge(acc);
+      if (lane == 0) err += pr_finish(v, acc, base, nindex, score, contrib_next);
+    }
+  } else {
+    const int w = threadIdx.x / kWave;
+    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+      Pair acc{0.0f, 0.0f};
+      for (int i = nindex[v] + (int)threadIdx.x; i < nindex[v + 1]; i += kBlock) two_sum_add(acc, contrib_cur[nlist[i]]);
+      acc = wave_merge(acc);
+      if (lane_id() == 0) s_pair[w] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        Pair t = s_pair[0];
+        for (int k = 1; k < kWavesPerBlock; k++) t = pair_merge(t, s_pair[k]);
+        err += pr_finish(v, t, base, nindex, score, contrib_next);
+      }
+      __syncthreads();
+    }
+  }
+  block_add_d(&ctl->res, err, s_red);
+  finish_round(ctl, [&] { pr_finalize_round(ctl, eps, max_iter); });
+}
+
+// ------------------------------------------------------------- balanced ----
+// Row classes are built once per run: medium rows go to a list; every large
+// row reserves a contiguous run of chunk slots (its kChunk-edge pieces in edge
+// order), so merging a row's partial sums in slot order is deterministic.
+struct BigRow {
+  int v;
+  int first;  // first chunk slot
+  int count;  // number of chunks
+};
+
+__global__ void pr_zero(int* __restrict__ p, int count)
+{
+  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) pr_classify(const int* __restrict__ nindex, int n, int* __restrict__ medium,
+                                                       int* __restrict__ counts, Entry* __restrict__ chunks,
+                                                       BigRow* __restrict__ big)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < n; base += stride) {
+    const int v = base + threadIdx.x;
+    int deg = 0, beg = 0, end = 0;
+    if (v < n) {
+      beg = nindex[v];
+      end = nindex[v + 1];
+      deg = end - beg;
+    }
+    const bool is_med = deg > kSmallRow && deg <= kMediumRow;
+    const int ms = wave_push_slot(&counts[0], is_med);
+    if (is_med) medium[ms] = v;
+    const bool is_big = deg > kMediumRow;
+    const int bs = wave_push_slot(&counts[1], is_big);
+    if (is_big) {
+      const int c = chunks_of(deg);
+      const int first = atomicAdd(&counts[2], c);
+      big[bs] = BigRow{v, first, c};
+      for (int k = 0; k < c; k++) {
+        const int b = beg + k * kChunk;
+        chunks[first + k] = Entry{v, b, (b + kChunk < end) ? b + kChunk : end, 0};
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) pr_bal_main(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                       int n, float base, float* __restrict__ score,
+                                                       const float* __restrict__ contrib_cur,
+                                                       float* __restrict__ contrib_next,
+                                                       const int* __restrict__ medium, const int* __restrict__ counts,
+                                                       const Entry* __restrict__ chunks, Pair* __restrict__ partial,
+                                                       Ctl* ctl)
+{
+  __shared__ double s_red[kWavesPerBlock];
+  if (round_done(ctl)) return;
+  double err = 0.0;
+  // small rows: one thread each
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const int beg = nindex[v], end = nindex[v + 1];
+    if (end - beg <= kSmallRow) {
+      Pair acc{0.0f, 0.0f};
+      for (int i = beg; i < end; i++) two_sum_add(acc, contrib_cur[nlist[i]]);
+      err += pr_finish(v, acc, base, nindex, score, contrib_next);
+    }
+  }
+  // medium rows: one wave each
+  const int waves = gridDim.x * kWavesPerBlock;
+  const int gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int nmed = counts[0];
+  for (int k = gw; k < nmed; k += waves) {
+    const int v = medium[k];
+    Pair acc{0.0f, 0.0f};
+    for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) two_sum_add(acc, contrib_cur[nlist[i]]);
+    acc = wave_merge(acc);
+    if (lane == 0) err += pr_finish(v, acc, base, nindex, score, contrib_next);
+  }
+  // chunks of large rows: one wave each, partial pairs to memory
+  const int nch = counts[2];
+  for (int k = gw; k < nch; k += waves) {
+    const Entry en = chunks[k];
+    Pair acc{0.0f, 0.0f};
+    for (int i = en.beg + lane; i < en.end; i += kWave) two_sum_add(acc, contrib_cur[nlist[i]]);
+    acc = wave_merge(acc);
+    if (lane == 0) partial[k] = acc;
+  }
+  block_add_d(&ctl->res, err, s_red);
+}
+
+__global__ void __launch_bounds__(kBlock) pr_bal_big(const int* __restrict__ nindex, float base,
+                                                      float* __restrict__ score, float* __restrict__ contrib_next,
+                                                      const BigRow* __restrict__ big, const int* __restrict__ counts,
+                                                      const Pair* __restrict__ partial, Ctl* ctl, double eps,
+                                                      int max_iter)
+{
+  __shared__ double s_red[kWavesPerBlock];
+  if (round_done(ctl)) return;
+  double err = 0.0;
+  const int nb = counts[1];
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nb; i += stride) {
+    const BigRow b = big[i];
+    Pair acc{0.0f, 0.0f};
+    for (int k = 0; k < b.count; k++) acc = pair_merge(acc, partial[b.first + k]);
+    err += pr_finish(b.v, acc, base, nindex, score, contrib_next);
+  }
+  block_add_d(&ctl->res, err, s_red);
+  finish_round(ctl, [&] { pr_finalize_round(ctl, eps, max_iter); });
+}
+
+// ------------------------------------------------------------ host side ----
+int pr_buffers(const igx_graph* g, float** c0, float** c1)
+{
+  void *p0 = nullptr, *p1 = nullptr;
+  IGX_TRY(ws_get(WS_F, sizeof(float) * (size_t)g->nodes, &p0));
+  IGX_TRY(ws_get(WS_G, sizeof(float) * (size_t)g->nodes, &p1));
+  *c0 = (float*)p0;
+  *c1 = (float*)p1;
+  return IGX_OK;
+}
+
+template <int G>
+int style_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  float* score = (float*)out;
+  float *c0 = nullptr, *c1 = nullptr;
+  IGX_TRY(pr_buffers(g, &c0, &c1));
+  const PrParams P = pr_params();
+  const float base = (1.0f - kDamp) / (float)g->nodes;  // pr_cuda.h:140
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(pr_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, g->nindex, g->nodes,
+                     score, c0);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long r) {
+        float* cur = (r & 1) ? c1 : c0;
+        float* nxt = (r & 1) ? c0 : c1;
+        hipLaunchKernelGGL((pr_pull<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                           g->nodes, base, score, cur, nxt, R.d_ctl, P.eps, P.max_iter);
+      },
+      P.max_iter, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  info->residual = R.h_ctl->res_last;
+  return rc == IGX_ERR_LIMIT ? IGX_OK : rc;
+}
+
+int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  float* score = (float*)out;
+  float *c0 = nullptr, *c1 = nullptr;
+  IGX_TRY(pr_buffers(g, &c0, &c1));
+  const int n = g->nodes;
+  const size_t nbig = (size_t)g->edges / kMediumRow + 64;
+  const size_t nchunks = (size_t)g->edges / kChunk + nbig;
+  void *pmed = nullptr, *pcnt = nullptr, *pch = nullptr, *pbig = nullptr, *ppart = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)n, &pmed));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * 64, &pcnt));
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * nchunks, &pch));
+  IGX_TRY(ws_get(WS_C, sizeof(BigRow) * nbig, &pbig));
+  IGX_TRY(ws_get(WS_H, sizeof(Pair) * nchunks, &ppart));
+  const PrParams P = pr_params();
+  const float base = (1.0f - kDamp) / (float)n;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(n));
+  hipLaunchKernelGGL(pr_zero, dim3(1), dim3(64), 0, R.stream, (int*)pcnt, 64);
+  hipLaunchKernelGGL(pr_init, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, R.stream, g->nindex, n, score, c0);
+  hipLaunchKernelGGL(pr_classify, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, n, (int*)pmed,
+                     (int*)pcnt, (Entry*)pch, (BigRow*)pbig);
+  const int big_grid = blocks_for((long long)nbig, kBlock);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long r) {
+        float* cur = (r & 1) ? c1 : c0;
+        float* nxt = (r & 1) ? c0 : c1;
+        hipLaunchKernelGGL(pr_bal_main, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n,
+                           base, score, cur, nxt, (const int*)pmed, (const int*)pcnt, (const Entry*)pch,
+                           (Pair*)ppart, R.d_ctl);
+        hipLaunchKernelGGL(pr_bal_big, dim3(big_grid), dim3(kBlock), 0, R.stream, g->nindex, base, score, nxt,
+                           (const BigRow*)pbig, (const int*)pcnt, (const Pair*)ppart, R.d_ctl, P.eps, P.max_iter);
+      },
+      P.max_iter, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  info->residual = R.h_ctl->res_last;
+  return rc == IGX_ERR_LIMIT ? IGX_OK : rc;
+}
+
+const StyleEntry kStyles[] = {
+    {"pull_thread_kahan", style_pull<kThread>, 0},
+    {"pull_wave_kahan", style_pull<kWaveG>, 0},
+    {"pull_block_kahan", style_pull<kBlockG>, 0},
+    {"pull_balanced_kahan", style_balanced, 0},
+};
+
+}  // namespace
+
+const StyleEntry* pr_styles(int* count)
+{
+  *count = (int)(sizeof(kStyles) / sizeof(kStyles[0]));
+  return kStyles;
+}
+
+}  // namespace igx

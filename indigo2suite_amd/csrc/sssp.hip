@@ -1,0 +1,457 @@
+// Single-source shortest paths on gfx950: int32 distances with the graph's
+// int32 edge weights (eweight), INT_MAX for unreachable vertices.  Every style
+// relaxes to the same fixed point (the shortest-path distances).
+//
+// Reference family: codes/cuda/sssp-cuda (168 variants).
+//   v_topo_push_*   [C-SVT] SSSP_CUDA_V_Topo_Push_..._Thread:56-120
+//   v_topo_pull_*   V_Topo_Pull siblings
+//   e_topo_push_*   E_Topo_Push siblings (one edge per item)
+//   v_data_push_*   V_Data_Push NonDup ([C-SVDw]:62-87 for the warp form)
+//   e_data_push_thread     [C-SED]:45-140 edge worklist, NonDup stamps on edges
+//   e_data_push_rw_thread  [C-SEDrw]:63-87 ReadWrite: racy store, the edge
+//                   itself is re-queued to repair a lost update
+//   v_data_push_balanced   (no reference counterpart) vertex worklist of edge
+//                   chunks expanded by wave-level merge path
+#include "igx_internal.hpp"
+#include "igx_frontier.hpp"
+
+namespace igx {
+namespace {
+
+enum Gran { kThread = 0, kWaveG = 1, kBlockG = 2 };
+
+__global__ void __launch_bounds__(kBlock) sssp_init(int src, int* __restrict__ dist, int n)
+{
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) dist[v] = (v == src) ? 0 : kInf;
+}
+
+__global__ void __launch_bounds__(kBlock) sssp_fill(int* __restrict__ p, int value, long long count)
+{
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) p[i] = value;
+}
+
+__device__ __forceinline__ void sssp_raise_flag(Ctl* ctl, bool updated)
+{
+  const uint64_t any = ballot(updated);
+  if (any != 0 && lane_id() == (__ffsll((unsigned long long)any) - 1)) store_relaxed(&ctl->flag, 1);
+}
+
+template <int G, int AK>
+__global__ void __launch_bounds__(kBlock) sssp_vtopo_push(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist,
+                                                           const int* __restrict__ ew, int n, int* __restrict__ dist,
+                                                           Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int s = dist[v];
+      if (s != kInf) {
+        for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+          const int nd = s + ew[i];
+          const int dst = nlist[i];
+          if (dist[dst] > nd && atomic_min_k<AK>(&dist[dst], nd) > nd) updated = true;
+        }
+      }
+    }
+  } else if constexpr (G == kWaveG) {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int s = dist[v];
+      if (s != kInf) {
+        for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) {
+          const int nd = s + ew[i];
+          const int dst = nlist[i];
+          if (dist[dst] > nd && atomic_min_k<AK>(&dist[dst], nd) > nd) updated = true;
+        }
+      }
+    }
+  } else {
+    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+      const int s = dist[v];
+      if (s != kInf) {
+        for (int i = nindex[v] + (int)threadIdx.x; i < nindex[v + 1]; i += kBlock) {
+          const int nd = s + ew[i];
+          const int dst = nlist[i];
+          if (dist[dst] > nd && atomic_min_k<AK>(&dist[dst], nd) > nd) updated = true;
+        }
+      }
+    }
+  }
+  sssp_raise_flag(ctl, updated);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+__global__ void __launch_bounds__(kBlock) sssp_vtopo_pull(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist,
+                                                           const int* __restrict__ ew, int n, int* __restrict__ dist,
+                                                           Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const int d0 = dist[v];
+    int d = d0;
+    for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+      const int s = load_relaxed(&dist[nlist[i]]);
+      if (s != kInf && s + ew[i] < d) d = s + ew[i];
+    }
+    if (d < d0) {
+      store_relaxed(&dist[v], d);
+      updated = true;
+    }
+  }
+  sssp_raise_flag(ctl, updated);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+__global__ void __launch_bounds__(kBlock) sssp_etopo_push(const int* __restrict__ nlist,
+                                                           const int* __restrict__ ew, const int* __restrict__ sp,
+                                                           int m, int* __restrict__ dist, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  const int stride = gridDim.x * kBlock;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+    const int s = dist[sp[e]];
+    if (s != kInf) {
+      const int nd = s + ew[e];
+      const int dst = nlist[e];
+      if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) updated = true;
+    }
+  }
+  sssp_raise_flag(ctl, updated);
+  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+// ------------------------------------------------------- vertex, data ----
+__global__ void sssp_vdata_seed(int src, int* __restrict__ wl, Ctl* ctl)
+{
+  if (threadIdx.x == 0) {
+    wl[0] = src;
+    ctl->wl_count[0] = 1;
+    ctl->wl_count[1] = 0;
+  }
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) sssp_vdata_push(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist,
+                                                           const int* __restrict__ ew, int* __restrict__ dist,
+                                                           int* __restrict__ wl0, int* __restrict__ wl1,
+                                                           int* __restrict__ stamp, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int p = iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int tag = iter + 1;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+      const int v = wl_in[idx];
+      const int s = dist[v];
+      for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+        const int nd = s + ew[i];
+        const int dst = nlist[i];
+        bool push = false;
+        if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) {
+          push = atomic_max_relaxed(&stamp[dst], tag) != tag;
+        }
+        const int slot = wave_push_slot(out_count, push);
+        if (push) wl_out[slot] = dst;
+      }
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int idx = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; idx < count; idx += waves) {
+      const int v = wl_in[idx];
+      const int s = dist[v];
+      const int beg = nindex[v], end = nindex[v + 1];
+      for (int base = beg; base < end; base += kWave) {
+        const int i = base + lane;
+        if (i < end) {
+          const int nd = s + ew[i];
+          const int dst = nlist[i];
+          bool push = false;
+          if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) {
+            push = atomic_max_relaxed(&stamp[dst], tag) != tag;
+          }
+          const int slot = wave_push_slot(out_count, push);
+          if (push) wl_out[slot] = dst;
+        }
+      }
+    }
+  }
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+// --------------------------------------------------------- edge, data ----
+__global__ void __launch_bounds__(kBlock) sssp_edata_seed(int src, const int* __restrict__ nindex,
+                                                           int* __restrict__ wl, Ctl* ctl)
+{
+  const int beg = nindex[src], end = nindex[src + 1];
+  for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) wl[i - beg] = i;
+  if (threadIdx.x == 0) {
+    ctl->wl_count[0] = end - beg;
+    ctl->wl_count[1] = 0;
+  }
+}
+
+template <bool RW>
+__global__ void __launch_bounds__(kBlock) sssp_edata_push(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist,
+                                                           const int* __restrict__ ew, const int* __restrict__ sp,
+                                                           int* __restrict__ dist, int* __restrict__ wl0,
+                                                           int* __restrict__ wl1, int* __restrict__ stamp, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int p = iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int tag = iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+    const int e = wl_in[idx];
+    const int s = load_relaxed(&dist[sp[e]]);
+    if (s != kInf) {
+      const int nd = s + ew[e];
+      const int dst = nlist[e];
+      bool improved;
+      if (RW) {
+        improved = load_relaxed(&dist[dst]) > nd;
+        if (improved) {
+          store_relaxed(&dist[dst], nd);
+          // a concurrent store may overwrite this one: re-queue e itself
+          const bool push = atomic_max_relaxed(&stamp[e], tag) != tag;
+          const int slot = wave_push_slot(out_count, push);
+          if (push) wl_out[slot] = e;
+        }
+      } else {
+        improved = dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd;
+      }
+      if (improved) {
+        for (int j = nindex[dst]; j < nindex[dst + 1]; j++) {
+          const bool push = atomic_max_relaxed(&stamp[j], tag) != tag;
+          const int slot = wave_push_slot(out_count, push);
+          if (push) wl_out[slot] = j;
+        }
+      }
+    }
+  }
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+// --------------------------------------------- balanced vertex worklist ----
+__global__ void __launch_bounds__(kBlock) sssp_bal_seed(int src, const int* __restrict__ nindex,
+                                                         Entry* __restrict__ q0, Ctl* ctl)
+{
+  const int beg = nindex[src], end = nindex[src + 1];
+  const int c = chunks_of(end - beg);
+  for (int k = threadIdx.x; k < c; k += kBlock) {
+    const int b = beg + k * kChunk;
+    q0[k] = Entry{src, b, (b + kChunk < end) ? b + kChunk : end, 0};
+  }
+  if (threadIdx.x == 0) {
+    ctl->wl_count[0] = c;
+    ctl->wl_count[1] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist,
+                                                          const int* __restrict__ ew, int* __restrict__ dist,
+                                                          Entry* __restrict__ q0, Entry* __restrict__ q1,
+                                                          int* __restrict__ stamp, Ctl* ctl)
+{
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ Entry s_ent[kWavesPerBlock][kWave];
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int p = iter & 1;
+  const Entry* q_in = p ? q1 : q0;
+  Entry* q_out = p ? q0 : q1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int qsize = ctl->wl_count[p];
+  const int tag = iter + 1;
+  const int w = threadIdx.x / kWave;
+  const int gw = blockIdx.x * kWavesPerBlock + w;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  wave_expand(q_in, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
+    const int s = dist[en.v];
+    const int nd = s + ew[e];
+    const int u = nlist[e];
+    bool push = false;
+    int ub = 0, ue = 0;
+    if (dist[u] > nd && atomic_min_relaxed(&dist[u], nd) > nd) {
+      push = atomic_max_relaxed(&stamp[u], tag) != tag;
+      if (push) {
+        ub = nindex[u];
+        ue = nindex[u + 1];
+      }
+    }
+    push_vertex_chunks(q_out, out_count, push && ue > ub, u, ub, ue);
+  });
+  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
+}
+
+// ------------------------------------------------------------ host side ----
+template <typename K>
+int run_sssp_rounds(const igx_graph* g, igx_run_info* info, long long max_rounds, K launch)
+{
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds([&](long long) { launch(); }, max_rounds, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  (void)g;
+  return rc;
+}
+
+// Bellman-Ford style rounds: at most n rounds change anything on a graph
+// with non-negative weights; data-driven rounds can revisit, so allow more.
+long long sssp_round_cap(const igx_graph* g) { return 4LL * g->nodes + 64; }
+
+template <int G, int AK>
+int style_vtopo_push(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
+    hipLaunchKernelGGL((sssp_vtopo_push<G, AK>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
+                       g->nlist, g->eweight, g->nodes, dist, R.d_ctl);
+  });
+}
+
+int style_vtopo_pull(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
+    hipLaunchKernelGGL(sssp_vtopo_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->eweight, g->nodes, dist, R.d_ctl);
+  });
+}
+
+int style_etopo_push(const igx_graph* g, int src, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const int* sp = (const int*)aux;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
+    hipLaunchKernelGGL(sssp_etopo_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, g->eweight, sp,
+                       g->edges, dist, R.d_ctl);
+  });
+}
+
+template <int G>
+int style_vdata_push(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const size_t items = (size_t)std::max(g->edges, g->nodes);
+  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * items, &p0));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * items, &p1));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * (size_t)g->nodes, &p2));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, (int*)p2, 0,
+                     (long long)g->nodes);
+  hipLaunchKernelGGL(sssp_vdata_seed, dim3(1), dim3(64), 0, R.stream, src, (int*)p0, R.d_ctl);
+  return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
+    hipLaunchKernelGGL((sssp_vdata_push<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->eweight, dist, (int*)p0, (int*)p1, (int*)p2, R.d_ctl);
+  });
+}
+
+template <bool RW>
+int style_edata_push(const igx_graph* g, int src, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const int* sp = (const int*)aux;
+  const size_t items = (size_t)std::max(g->edges, g->nodes);
+  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * items, &p0));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * items, &p1));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * items, &p2));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for((long long)items, kBlock)), dim3(kBlock), 0, R.stream, (int*)p2, 0,
+                     (long long)items);
+  hipLaunchKernelGGL(sssp_edata_seed, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, (int*)p0, R.d_ctl);
+  return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
+    hipLaunchKernelGGL((sssp_edata_push<RW>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->eweight, sp, dist, (int*)p0, (int*)p1, (int*)p2, R.d_ctl);
+  });
+}
+
+int style_balanced(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const size_t cap = (size_t)g->nodes + (size_t)g->edges / kChunk + 64;
+  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &p0));
+  IGX_TRY(ws_get(WS_E, sizeof(Entry) * cap, &p1));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * (size_t)g->nodes, &p2));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, (int*)p2, 0,
+                     (long long)g->nodes);
+  hipLaunchKernelGGL(sssp_bal_seed, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, (Entry*)p0, R.d_ctl);
+  return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
+    hipLaunchKernelGGL(sssp_bal_round, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->eweight, dist, (Entry*)p0, (Entry*)p1, (int*)p2, R.d_ctl);
+  });
+}
+
+const StyleEntry kStyles[] = {
+    {"v_topo_push_thread", style_vtopo_push<kThread, kAtomicRelaxed>, 0},
+    {"v_topo_push_wave", style_vtopo_push<kWaveG, kAtomicRelaxed>, 0},
+    {"v_topo_push_block", style_vtopo_push<kBlockG, kAtomicRelaxed>, 0},
+    {"v_topo_push_thread_cas", style_vtopo_push<kThread, kAtomicCas>, 0},
+    {"v_topo_pull_thread", style_vtopo_pull, 0},
+    {"e_topo_push_thread", style_etopo_push, 1},
+    {"v_data_push_thread", style_vdata_push<kThread>, 0},
+    {"v_data_push_wave", style_vdata_push<kWaveG>, 0},
+    {"e_data_push_thread", style_edata_push<false>, 1},
+    {"e_data_push_rw_thread", style_edata_push<true>, 1},
+    {"v_data_push_balanced", style_balanced, 0},
+};
+
+}  // namespace
+
+const StyleEntry* sssp_styles(int* count)
+{
+  *count = (int)(sizeof(kStyles) / sizeof(kStyles[0]));
+  return kStyles;
+}
+
+}  // namespace igx

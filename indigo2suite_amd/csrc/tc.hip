@@ -1,0 +1,349 @@
+// Triangle counting on gfx950: every triangle counted once, in uint64.
+// Requires strictly increasing adjacency (sorted, no duplicates, no self
+// loops), which the entry checks first as tc_vertex_cuda.h:157-165 does.
+//
+// Reference family: codes/cuda/tc-cuda (72 variants).
+//   v_merge_thread   [C-TV] TC_V_Thread_..._GlobalAdd:42-59 with d_common
+//                    (tc_vertex_cuda.h:44-63): for v and each neighbour u > v,
+//                    merge-count the neighbours above u common to both lists
+//   v_merge_wave     TC_V_Warp siblings: one wave per vertex, lanes take the u
+//   e_bsearch_thread [C-TE]:42-61 with d_find (tc_edge_cuda.h:116-127): for each
+//                    edge src > dst, look every u < dst of N(dst) up in N(src)
+//                    (exclusive upper bound here; the reference's inclusive
+//                    bound reads one past the list)
+//   oriented_lds     (no reference counterpart) degree-ordered orientation:
+//                    each edge points from lower to higher (degree, id) rank,
+//                    so hub lists become short; one wave per vertex stages its
+//                    out-list in LDS and every lane probes it for the
+//                    out-neighbours of one out-neighbour
+// Counts are reduced per wave and added to one uint64 with a single atomic
+// per wave (the reference's GlobalAdd/BlockAdd/Reduction axis).
+#include "igx_internal.hpp"
+#include "igx_scan.hpp"
+
+namespace igx {
+namespace {
+
+__global__ void tc_zero(unsigned long long* __restrict__ c, int* __restrict__ flag)
+{
+  if (threadIdx.x == 0) {
+    if (c) *c = 0ull;
+    if (flag) *flag = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) tc_check_sorted(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                           int n, int* __restrict__ bad)
+{
+  const int waves = gridDim.x * kWavesPerBlock;
+  const int lane = lane_id();
+  bool b = false;
+  for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+    for (int i = nindex[v] + 1 + lane; i < nindex[v + 1]; i += kWave) {
+      if (nlist[i - 1] >= nlist[i]) b = true;
+    }
+  }
+  if (ballot(b) != 0 && lane == 0) store_relaxed(bad, 1);
+}
+
+__device__ __forceinline__ void wave_add_count(unsigned long long* out, unsigned long long c)
+{
+  #pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
+  if (lane_id() == 0 && c != 0) atomicAdd(out, c);
+}
+
+// first index in [beg, end) whose value is > key (sorted list)
+__device__ __forceinline__ int upper_pos(const int* __restrict__ nlist, int beg, int end, int key)
+{
+  while (beg < end) {
+    const int mid = (beg + end) >> 1;
+    if (nlist[mid] <= key) beg = mid + 1;
+    else end = mid;
+  }
+  return beg;
+}
+
+// tc_vertex_cuda.h:44-63 merge count of two sorted ranges
+__device__ __forceinline__ int merge_common(int b1, int e1, int b2, int e2, const int* __restrict__ nlist)
+{
+  int c = 0;
+  while (b1 < e1 && b2 < e2) {
+    const int x = nlist[b1], y = nlist[b2];
+    if (x < y) b1++;
+    else if (y < x) b2++;
+    else {
+      c++;
+      b1++;
+      b2++;
+    }
+  }
+  return c;
+}
+
+template <bool WAVE>
+__global__ void __launch_bounds__(kBlock) tc_vertex_merge(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist, int n,
+                                                           unsigned long long* __restrict__ count)
+{
+  unsigned long long c = 0;
+  if constexpr (!WAVE) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int beg1 = nindex[v], end1 = nindex[v + 1];
+      const int start1 = upper_pos(nlist, beg1, end1, v);
+      for (int j = start1; j < end1; j++) {
+        const int u = nlist[j];
+        const int beg2 = nindex[u], end2 = nindex[u + 1];
+        const int start2 = upper_pos(nlist, beg2, end2, u);
+        c += (unsigned long long)merge_common(j + 1, end1, start2, end2, nlist);
+      }
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int beg1 = nindex[v], end1 = nindex[v + 1];
+      const int start1 = upper_pos(nlist, beg1, end1, v);
+      for (int j = start1 + lane; j < end1; j += kWave) {
+        const int u = nlist[j];
+        const int beg2 = nindex[u], end2 = nindex[u + 1];
+        const int start2 = upper_pos(nlist, beg2, end2, u);
+        c += (unsigned long long)merge_common(j + 1, end1, start2, end2, nlist);
+      }
+    }
+  }
+  wave_add_count(count, c);
+}
+
+__global__ void __launch_bounds__(kBlock) tc_edge_bsearch(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                           int m, unsigned long long* __restrict__ count)
+{
+  unsigned long long c = 0;
+  const int stride = gridDim.x * kBlock;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+    const int src = sp[e];
+    const int dst = nlist[e];
+    if (src > dst) {
+      const int beg2 = nindex[src], end2 = nindex[src + 1];
+      for (int i = nindex[dst]; i < nindex[dst + 1]; i++) {
+        const int u = nlist[i];
+        if (u >= dst) break;
+        int lo = beg2, hi = end2;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          const int x = nlist[mid];
+          if (x == u) {
+            c++;
+            break;
+          }
+          if (x < u) lo = mid + 1;
+          else hi = mid;
+        }
+      }
+    }
+  }
+  wave_add_count(count, c);
+}
+
+// ------------------------------------------------------------- oriented ----
+__device__ __forceinline__ bool rank_above(int du, int u, int dv, int v)
+{
+  return (du > dv) || (du == dv && u > v);
+}
+
+__global__ void __launch_bounds__(kBlock) tc_orient_count(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist, int n,
+                                                           int* __restrict__ ocount)
+{
+  const int waves = gridDim.x * kWavesPerBlock;
+  const int lane = lane_id();
+  for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+    const int beg = nindex[v], end = nindex[v + 1];
+    const int dv = end - beg;
+    int c = 0;
+    for (int i = beg + lane; i < end; i += kWave) {
+      const int u = nlist[i];
+      c += rank_above(nindex[u + 1] - nindex[u], u, dv, v) ? 1 : 0;
+    }
+    c = wave_sum(c);
+    if (lane == 0) ocount[v] = c;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) tc_orient_fill(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int n,
+                                                          const int* __restrict__ onidx, int* __restrict__ olist)
+{
+  const int waves = gridDim.x * kWavesPerBlock;
+  const int lane = lane_id();
+  for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+    const int beg = nindex[v], end = nindex[v + 1];
+    const int dv = end - beg;
+    int out = onidx[v];
+    for (int base = beg; base < end; base += kWave) {
+      const int i = base + lane;
+      bool keep = false;
+      int u = 0;
+      if (i < end) {
+        u = nlist[i];
+        keep = rank_above(nindex[u + 1] - nindex[u], u, dv, v);
+      }
+      const uint64_t mask = ballot(keep);
+      if (keep) olist[out + mbcnt(mask)] = u;
+      out += __popcll(mask);
+    }
+  }
+}
+
+constexpr int kLdsList = 1024;  // out-list entries staged per wave
+
+__global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restrict__ onidx,
+                                                             const int* __restrict__ olist, int n,
+                                                             unsigned long long* __restrict__ count)
+{
+  __shared__ int s_list[kWavesPerBlock][kLdsList];
+  const int w = threadIdx.x / kWave;
+  const int waves = gridDim.x * kWavesPerBlock;
+  const int lane = lane_id();
+  int* lds = s_list[w];
+  unsigned long long c = 0;
+  for (int v = blockIdx.x * kWavesPerBlock + w; v < n; v += waves) {
+    const int beg = onidx[v], end = onidx[v + 1];
+    const int len = end - beg;
+    if (len < 2) continue;
+    const bool staged = len <= kLdsList;
+    if (staged) {
+      for (int i = lane; i < len; i += kWave) lds[i] = olist[beg + i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int j = beg + lane; j < end; j += kWave) {
+      const int u = olist[j];
+      for (int k = onidx[u]; k < onidx[u + 1]; k++) {
+        const int x = olist[k];
+        int lo = 0, hi = len;
+        bool hit = false;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          const int y = staged ? lds[mid] : olist[beg + mid];
+          if (y == x) {
+            hit = true;
+            break;
+          }
+          if (y < x) lo = mid + 1;
+          else hi = mid;
+        }
+        c += hit ? 1u : 0u;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+  wave_add_count(count, c);
+}
+
+// ------------------------------------------------------------ host side ----
+int check_sorted(const igx_graph* g)
+{
+  Runtime& R = rt();
+  void* p = nullptr;
+  IGX_TRY(ws_get(WS_H, sizeof(int) * 64, &p));
+  int* bad = (int*)p;
+  hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, (unsigned long long*)nullptr, bad);
+  hipLaunchKernelGGL(tc_check_sorted, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes,
+                     bad);
+  int h = 0;
+  IGX_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, R.stream));
+  IGX_HIP(hipStreamSynchronize(R.stream));
+  return h ? IGX_ERR_UNSORTED : IGX_OK;
+}
+
+template <bool WAVE>
+int style_vertex(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  unsigned long long* count = (unsigned long long*)out;
+  IGX_TRY(check_sorted(g));
+  IGX_TRY(timer_start());
+  hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
+  hipLaunchKernelGGL((tc_vertex_merge<WAVE>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                     g->nodes, count);
+  IGX_HIP(hipGetLastError());
+  IGX_TRY(timer_stop(&info->seconds));
+  IGX_HIP(hipMemcpy(&info->count, count, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  info->iterations = 1;
+  info->launches = 2;
+  return IGX_OK;
+}
+
+int style_edge(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  unsigned long long* count = (unsigned long long*)out;
+  const int* sp = (const int*)aux;
+  IGX_TRY(check_sorted(g));
+  IGX_TRY(timer_start());
+  hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
+  hipLaunchKernelGGL(tc_edge_bsearch, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, sp,
+                     g->edges, count);
+  IGX_HIP(hipGetLastError());
+  IGX_TRY(timer_stop(&info->seconds));
+  IGX_HIP(hipMemcpy(&info->count, count, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  info->iterations = 1;
+  info->launches = 2;
+  return IGX_OK;
+}
+
+int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  unsigned long long* count = (unsigned long long*)out;
+  const int n = g->nodes;
+  IGX_TRY(check_sorted(g));
+  void *pcnt = nullptr, *pidx = nullptr, *plist = nullptr, *ptile = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * ((size_t)n + 1), &pcnt));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * ((size_t)n + 1), &pidx));
+  // an undirected graph has exactly edges/2 oriented edges; size for edges so
+  // that an asymmetric input cannot overrun the list
+  IGX_TRY(ws_get(WS_C, sizeof(int) * ((size_t)g->edges + 64), &plist));
+  IGX_TRY(ws_get(WS_E, sizeof(int) * ((size_t)scan_tiles_for(n + 1) + 64), &ptile));
+  int* ocount = (int*)pcnt;
+  int* onidx = (int*)pidx;
+  int* tiles = (int*)ptile;
+  IGX_TRY(timer_start());
+  hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
+  hipLaunchKernelGGL(tc_orient_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n,
+                     ocount);
+  // scan n+1 entries so that onidx[n] is the total; ocount[n] is zeroed first
+  hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, (unsigned long long*)nullptr, ocount + n);
+  exclusive_scan(ocount, onidx, n + 1, tiles, tiles + scan_tiles_for(n + 1), R.stream);
+  hipLaunchKernelGGL(tc_orient_fill, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, onidx,
+                     (int*)plist);
+  hipLaunchKernelGGL(tc_oriented_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, onidx, (const int*)plist, n,
+                     count);
+  IGX_HIP(hipGetLastError());
+  IGX_TRY(timer_stop(&info->seconds));
+  IGX_HIP(hipMemcpy(&info->count, count, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  info->iterations = 1;
+  info->launches = 8;
+  return IGX_OK;
+}
+
+const StyleEntry kStyles[] = {
+    {"v_merge_thread", style_vertex<false>, 0},
+    {"v_merge_wave", style_vertex<true>, 0},
+    {"e_bsearch_thread", style_edge, 1},
+    {"oriented_lds", style_oriented, 0},
+};
+
+}  // namespace
+
+const StyleEntry* tc_styles(int* count)
+{
+  *count = (int)(sizeof(kStyles) / sizeof(kStyles[0]));
+  return kStyles;
+}
+
+}  // namespace igx

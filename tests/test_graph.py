@@ -1,0 +1,141 @@
+"""Host graph substrate: .egr I/O (ECLgraph.h:52-106 semantics), generators, weights."""
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from indigo2suite_amd import graph as G
+from indigo2suite_amd._native import IgxError
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _edge_set(g):
+    src = g.sp()
+    return set(zip(src.tolist(), g.nlist.tolist()))
+
+
+def _assert_clean_undirected(g):
+    assert g.nindex[0] == 0 and g.nindex[-1] == g.edges
+    assert np.all(np.diff(g.nindex) >= 0)
+    assert g.is_sorted()  # strictly increasing => no duplicates, no self loops
+    src = g.sp()
+    assert not np.any(src == g.nlist)
+    # symmetric: the sorted (u, v) pairs equal the sorted (v, u) pairs
+    fwd = src.astype(np.int64) * g.nodes + g.nlist
+    bwd = g.nlist.astype(np.int64) * g.nodes + src
+    assert np.array_equal(np.sort(fwd), np.sort(bwd))
+
+
+@pytest.mark.parametrize("make", [
+    lambda: G.erdos_renyi(2000, 8, seed=3),
+    lambda: G.rmat(12, 16, seed=2),
+    lambda: G.rmat(12, 8, seed=2, n_limit=3000),
+    lambda: G.grid2d(40, 0.05, seed=9),
+    lambda: G.chung_lu(5000, 12, 2.5, seed=4),
+])
+def test_generators_clean_symmetric_sorted(make):
+    _assert_clean_undirected(make())
+
+
+def test_generators_deterministic_and_seeded():
+    a = G.rmat(11, 16, seed=5)
+    b = G.rmat(11, 16, seed=5)
+    c = G.rmat(11, 16, seed=6)
+    assert np.array_equal(a.nindex, b.nindex) and np.array_equal(a.nlist, b.nlist)
+    assert not (a.edges == c.edges and np.array_equal(a.nlist, c.nlist))
+
+
+def test_generators_independent_of_thread_count():
+    code = ("import sys; sys.path.insert(0, %r); from indigo2suite_amd import graph as G; import hashlib;"
+            "g = G.rmat(13, 16, seed=9); h = G.erdos_renyi(3000, 6, seed=2);"
+            "print(hashlib.sha1(g.nlist.tobytes() + h.nlist.tobytes()).hexdigest())" % REPO)
+    outs = set()
+    for threads in ("1", "3", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=threads)
+        outs.add(subprocess.check_output([sys.executable, "-c", code], env=env, text=True).strip())
+    assert len(outs) == 1
+
+
+def test_er_degree_and_grid_shape():
+    g = G.erdos_renyi(16384, 8, seed=1)
+    assert g.nodes == 16384
+    assert abs(g.edges / g.nodes - 8.0) < 0.2  # expected 2 * 65536 directed entries
+    side = 64
+    grid = G.grid2d(side, 0.0, seed=1)
+    assert grid.edges == 2 * 2 * side * (side - 1)
+    gl = G.grid2d(side, 0.05, seed=1)
+    extra = (gl.edges - grid.edges) // 2
+    assert 0.02 * side * side < extra < 0.08 * side * side
+
+
+def test_rmat_n_limit_drops_out_of_range():
+    g = G.rmat(10, 16, seed=1, n_limit=700)
+    assert g.nodes == 700 and g.nlist.max() < 700
+
+
+def test_weights_symmetric_in_range():
+    g = G.rmat(11, 16, seed=1).with_weights(1, 255, seed=1)
+    assert g.eweight.min() >= 1 and g.eweight.max() <= 255
+    src = g.sp()
+    w = {}
+    for u, v, x in zip(src.tolist(), g.nlist.tolist(), g.eweight.tolist()):
+        w[(u, v)] = x
+    assert all(w[(v, u)] == x for (u, v), x in w.items())
+
+
+def test_reference_fallback_weights_match_formula():
+    g = G.erdos_renyi(300, 6, seed=2).with_reference_weights()
+    src = g.sp().astype(np.int64)
+    prod = (src * g.nlist.astype(np.int64)) & 0xFFFFFFFF
+    prod = np.where(prod >= 2**31, prod - 2**32, prod)  # int32 wrap
+    expect = np.abs(1 + np.fmod(prod, g.nodes))
+    assert np.array_equal(g.eweight, expect.astype(np.int32))
+
+
+def test_egr_roundtrip(tmp_path):
+    g = G.rmat(9, 8, seed=1)
+    p = str(tmp_path / "g.egr")
+    g.write_egr(p)
+    h = G.read_egr(p)
+    assert h.eweight is None
+    assert np.array_equal(g.nindex, h.nindex) and np.array_equal(g.nlist, h.nlist)
+    gw = g.with_weights()
+    gw.write_egr(p)
+    h = G.read_egr(p)
+    assert np.array_equal(h.eweight, gw.eweight)
+    # byte layout: int32 nodes, int32 edges, nindex, nlist, eweight
+    raw = open(p, "rb").read()
+    n, m = struct.unpack("<ii", raw[:8])
+    assert (n, m) == (g.nodes, g.edges)
+    assert len(raw) == 8 + 4 * (n + 1) + 8 * m
+
+
+def test_egr_errors(tmp_path):
+    with pytest.raises(IgxError) as e:
+        G.read_egr(str(tmp_path / "missing.egr"))
+    assert e.value.status == 2
+    bad = tmp_path / "trunc.egr"
+    g = G.rmat(8, 8, seed=1)
+    g.write_egr(str(bad))
+    data = bad.read_bytes()
+    bad.write_bytes(data[: 8 + 4 * (g.nodes + 1) + 10])
+    with pytest.raises(IgxError) as e:
+        G.read_egr(str(bad))
+    assert e.value.status == 2
+    zero = tmp_path / "zero.egr"
+    zero.write_bytes(struct.pack("<ii", 0, 0))
+    with pytest.raises(IgxError) as e:
+        G.read_egr(str(zero))
+    assert e.value.status == 3
+
+
+def test_features_shape_and_values():
+    f = G.rmat(12, 16, seed=1).features()
+    assert f.shape == (16,) and np.all(np.isfinite(f))
+    assert f[0] == 12.0
+    grid = G.grid2d(32, 0.0, seed=1).features()
+    assert grid[7] == 0.0 and abs(grid[8] - np.log2(1 + 62)) < 1e-9  # grid diameter 2*(32-1)
