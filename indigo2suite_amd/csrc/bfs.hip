@@ -556,8 +556,10 @@ int style_do_hybrid(const igx_graph* g, int src, void* out, const void*, igx_run
   int rounds = 0, launches = 0;
   const int rc = run_rounds(
       [&](long long) {
+        ktimer_begin("bfs_do_level");
         hipLaunchKernelGGL(bfs_do_level, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                            g->nodes, dist, (Entry*)p0, (Entry*)p1, R.d_ctl);
+        ktimer_end("bfs_do_level");
       },
       (long long)g->nodes + 2, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));

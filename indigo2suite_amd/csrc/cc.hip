@@ -421,8 +421,10 @@ int style_uf(const igx_graph* g, int, void* out, const void*, igx_run_info* info
   hipLaunchKernelGGL(cc_fill, dim3(1), dim3(kBlock), 0, R.stream, (int*)pc, 0, 1LL);
   hipLaunchKernelGGL(uf_init, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent,
                      (Entry*)pq, (int*)pc);
+  ktimer_begin("uf_hook");
   hipLaunchKernelGGL(uf_hook, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent,
                      (const Entry*)pq, (const int*)pc);
+  ktimer_end("uf_hook");
   hipLaunchKernelGGL(uf_compress, dim3(grid), dim3(kBlock), 0, R.stream, g->nodes, parent);
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));

@@ -127,4 +127,10 @@ int device_sp(const igx_graph* g, const int** d_sp);
 int timer_start();
 int timer_stop(double* seconds);
 
+// Per-kernel timing for the roofline: when igx_profile_kernel() selected
+// `tag`, launches wrapped in ktimer_begin/ktimer_end with that tag are
+// bracketed by events on the library stream and their durations summed.
+void ktimer_begin(const char* tag);
+void ktimer_end(const char* tag);
+
 }  // namespace igx

@@ -264,8 +264,10 @@ int style_vdata_pull(const igx_graph* g, int, void* out, const void*, igx_run_in
   hipLaunchKernelGGL(mis_seed, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, (int*)p0, g->nodes,
                      R.d_ctl);
   return mis_rounds(g, info, [&] {
+    ktimer_begin("mis_vdata_pull");
     hipLaunchKernelGGL(mis_vdata_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        status, (int*)p0, (int*)p1, R.d_ctl);
+    ktimer_end("mis_vdata_pull");
   });
 }
 

@@ -340,9 +340,11 @@ int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info
       [&](long long r) {
         float* cur = (r & 1) ? c1 : c0;
         float* nxt = (r & 1) ? c0 : c1;
+        ktimer_begin("pr_bal_main");
         hipLaunchKernelGGL(pr_bal_main, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n,
                            base, score, cur, nxt, (const int*)pmed, (const int*)pcnt, (const Entry*)pch,
                            (Pair*)ppart, R.d_ctl);
+        ktimer_end("pr_bal_main");
         hipLaunchKernelGGL(pr_bal_big, dim3(big_grid), dim3(kBlock), 0, R.stream, g->nindex, base, score, nxt,
                            (const BigRow*)pbig, (const int*)pcnt, (const Pair*)ppart, R.d_ctl, P.eps, P.max_iter);
       },

@@ -20,6 +20,12 @@ PrParams g_pr;
 int g_entry_style[IGX_NUM_ALGOS][2];
 bool g_entry_style_set = false;
 
+// per-kernel event pairs (ktimer_begin / ktimer_end)
+char g_prof_tag[64] = "";
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
+size_t g_prof_used = 0;
+bool g_prof_open = false;
+
 // cached device sp[] keyed by the nlist pointer and edge count
 const int* g_sp_key = nullptr;
 int g_sp_edges = -1;
@@ -106,6 +112,25 @@ int device_sp(const igx_graph* g, const int** d_sp)
   g_sp_edges = g->edges;
   *d_sp = (const int*)p;
   return IGX_OK;
+}
+
+void ktimer_begin(const char* tag)
+{
+  if (g_prof_tag[0] == 0 || strcmp(tag, g_prof_tag) != 0 || g_prof_open) return;
+  if (g_prof_used == g_prof_events.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    g_prof_events.push_back({a, b});
+  }
+  if (hipEventRecord(g_prof_events[g_prof_used].first, g_rt.stream) == hipSuccess) g_prof_open = true;
+}
+
+void ktimer_end(const char* tag)
+{
+  if (!g_prof_open || strcmp(tag, g_prof_tag) != 0) return;
+  hipEventRecord(g_prof_events[g_prof_used].second, g_rt.stream);
+  g_prof_used++;
+  g_prof_open = false;
 }
 
 int timer_start()
@@ -393,6 +418,36 @@ int igx_set_entry_style(int algo, int edge_kind, int style)
   if (style < 0 || style >= igx_num_styles(algo)) return IGX_ERR_STYLE;
   init_entry_styles();
   g_entry_style[algo][edge_kind ? 1 : 0] = style;
+  return IGX_OK;
+}
+
+int igx_profile_kernel(const char* tag)
+{
+  IGX_TRY(ensure_init());
+  IGX_HIP(hipStreamSynchronize(g_rt.stream));
+  g_prof_used = 0;
+  g_prof_open = false;
+  if (tag == nullptr) {
+    g_prof_tag[0] = 0;
+  } else {
+    snprintf(g_prof_tag, sizeof(g_prof_tag), "%s", tag);
+  }
+  return IGX_OK;
+}
+
+int igx_profile_read(double* total_ms, int* launches)
+{
+  IGX_TRY(ensure_init());
+  IGX_HIP(hipStreamSynchronize(g_rt.stream));
+  double sum = 0.0;
+  for (size_t i = 0; i < g_prof_used; i++) {
+    float ms = 0.0f;
+    IGX_HIP(hipEventElapsedTime(&ms, g_prof_events[i].first, g_prof_events[i].second));
+    sum += ms;
+  }
+  if (total_ms) *total_ms = sum;
+  if (launches) *launches = (int)g_prof_used;
+  g_prof_used = 0;
   return IGX_OK;
 }
 

@@ -427,8 +427,10 @@ int style_balanced(const igx_graph* g, int src, void* out, const void*, igx_run_
                      (long long)g->nodes);
   hipLaunchKernelGGL(sssp_bal_seed, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, (Entry*)p0, R.d_ctl);
   return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
+    ktimer_begin("sssp_bal_round");
     hipLaunchKernelGGL(sssp_bal_round, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        g->eweight, dist, (Entry*)p0, (Entry*)p1, (int*)p2, R.d_ctl);
+    ktimer_end("sssp_bal_round");
   });
 }
 

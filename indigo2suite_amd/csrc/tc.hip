@@ -321,8 +321,10 @@ int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info
   exclusive_scan(ocount, onidx, n + 1, tiles, tiles + scan_tiles_for(n + 1), R.stream);
   hipLaunchKernelGGL(tc_orient_fill, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, onidx,
                      (int*)plist);
+  ktimer_begin("tc_oriented_count");
   hipLaunchKernelGGL(tc_oriented_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, onidx, (const int*)plist, n,
                      count);
+  ktimer_end("tc_oriented_count");
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
   IGX_HIP(hipMemcpy(&info->count, count, sizeof(unsigned long long), hipMemcpyDeviceToHost));

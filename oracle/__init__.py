@@ -52,6 +52,7 @@ def lib():
         sig("orc_bfs_omp_topo_pull", ctypes.c_double, [ctypes.c_int, G, ip, ip])
         sig("orc_sssp_serial", None, [ctypes.c_int, G, ip])
         sig("orc_sssp_omp_edge_data", ctypes.c_double, [ctypes.c_int, G, ip, ip, ip])
+        sig("orc_sssp_omp_topo_pull", ctypes.c_double, [ctypes.c_int, G, ip, ip])
         sig("orc_cc_omp_topo_push", ctypes.c_double, [G, ip, ip])
         sig("orc_cc_omp_topo_pull", ctypes.c_double, [G, ip, ip])
         sig("orc_cc_verify", ctypes.c_int, [G, ip])
@@ -110,6 +111,13 @@ def sssp_omp_edge_data(g, src):
     sp = np.ascontiguousarray(g.sp(), np.int32)
     it = ctypes.c_int()
     t = lib().orc_sssp_omp_edge_data(src, ctypes.byref(_gs(g)), _ip(sp), _ip(out), ctypes.byref(it))
+    return out, t, it.value
+
+
+def sssp_omp_pull(g, src):
+    out = np.empty(g.nodes, np.int32)
+    it = ctypes.c_int()
+    t = lib().orc_sssp_omp_topo_pull(src, ctypes.byref(_gs(g)), _ip(out), ctypes.byref(it))
     return out, t, it.value
 
 

@@ -11,6 +11,7 @@
 //   [O-BFS]     codes/omp/bfs-omp/BFS_OMP_V_Topo_Push_IntType_NoRangeBug_ReadModifyWrite_Default_NoBoundsBug_Atomic_NonDeterm.cpp
 //   [O-BFSpull] codes/omp/bfs-omp/BFS_OMP_V_Topo_Pull_IntType_NoRangeBug_Default_NoBoundsBug_Atomic_NonDeterm.cpp
 //   [O-SSSP]    codes/omp/sssp-omp/SSSP_OMP_E_Data_Push_NonDeterm_IntType_ReadModifyWrite_Atomic_NonDup_NoFieldBug_Default.cpp
+//   [O-SSSPpull] codes/omp/sssp-omp/SSSP_OMP_V_Topo_Pull_IntType_NonDeterm_Default_NoBoundsBug_Atomic_NoRangeBug.cpp
 //   [O-CC]      codes/omp/cc-omp/CC_OMP_V_Topo_Push_IntType_ReadModifyWrite_NonDeterm_Default_NoBoundsBug_Atomic.cpp
 //   [O-CCpull]  codes/omp/cc-omp/CC_OMP_V_Topo_Pull_IntType_NonDeterm_Default_NoBoundsBug_Atomic.cpp
 //   [O-MIS]     codes/omp/mis-omp/MIS_OMP_V_Topo_Push_IntType_Atomic_NonDeterm_Default_NoBoundsBug.cpp
@@ -295,6 +296,40 @@ double orc_sssp_omp_edge_data(int src, const igx_graph* g, const int* sp, int* d
     std::swap(w1, w2);
     wl1size = wl2size;
   } while (wl1size > 0);
+  const double t = omp_get_wtime() - t0;
+  if (iters) *iters = iter;
+  return t;
+}
+
+// [O-SSSPpull] codes/omp/sssp-omp/SSSP_OMP_V_Topo_Pull_IntType_NonDeterm_Default_NoBoundsBug_Atomic_NoRangeBug.cpp
+// :41-90 -- vertex-based, topology-driven pull (the fastest OpenMP SSSP form).
+double orc_sssp_omp_topo_pull(int src, const igx_graph* g, int* dist, int* iters)
+{
+  const int n = g->nodes;
+  for (int v = 0; v < n; v++) dist[v] = (v == src) ? 0 : INT_MAX;
+  const double t0 = omp_get_wtime();
+  int goagain;
+  int iter = 0;
+  do {
+    iter++;
+    goagain = 0;
+    #pragma omp parallel for
+    for (int v = 0; v < n; v++) {  // :51-71
+      const int beg = g->nindex[v], end = g->nindex[v + 1];
+      int d = atomic_read(&dist[v]);
+      for (int i = beg; i < end; i++) {
+        const int s = atomic_read(&dist[g->nlist[i]]);
+        if (s != INT_MAX) {
+          const int new_dist = s + g->eweight[i];
+          if (new_dist < d) {
+            d = new_dist;
+            atomic_write(&goagain, 1);
+          }
+        }
+        atomic_write(&dist[v], d);
+      }
+    }
+  } while (goagain);
   const double t = omp_get_wtime() - t0;
   if (iters) *iters = iter;
   return t;

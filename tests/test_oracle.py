@@ -91,6 +91,7 @@ def test_oracle_matches_reference_golden(name):
     assert np.array_equal(O.bfs_omp_pull(g, src)[0], d["bfs_pull"])
     assert np.array_equal(O.sssp_serial(g, src), d["sssp"])
     assert np.array_equal(O.sssp_omp_edge_data(g, src)[0], d["sssp"])
+    assert np.array_equal(O.sssp_omp_pull(g, src)[0], d["sssp"])
     assert np.array_equal(O.cc_omp_push(g)[0], d["cc"])
     assert np.array_equal(O.cc_omp_pull(g)[0], d["cc"])
     assert np.array_equal(O.mis_omp_push(g)[0], d["mis"])
