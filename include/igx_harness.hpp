@@ -31,6 +31,7 @@ namespace igx_harness {
 // readECLgraph (ECLgraph.h:52-80) behaviour: print the counts, exit(-1) on error
 inline igx_graph read_graph(const char* path)
 {
+  setvbuf(stdout, nullptr, _IOLBF, 0);
   igx_graph g = {};
   const int rc = igx_egr_read(path, &g);
   if (rc != IGX_OK) {
@@ -52,6 +53,8 @@ inline void check(int rc, const char* what)
 // GPUinfo (bfs_vertex_cuda.h:63-73): select the device and print it
 inline int gpu_info(int d)
 {
+  setvbuf(stdout, nullptr, _IOLBF, 0);  // keep lines visible if the program dies
+
   check(igx_init(d), "igx_init");
   char name[256];
   int cus = 0, mt = 0;

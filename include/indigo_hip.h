@@ -150,6 +150,52 @@ int igx_tc_vertex(unsigned long long* count, int nodes, const int* d_nindex, con
 int igx_tc_edge(unsigned long long* count, int edges, const int* d_nindex, const int* d_nlist, const int* d_sp,
                 double* seconds);
 
+/* ---------------------------------------------------------- batched mode ---- */
+/* No reference counterpart (the reference runs one program per graph).  Many
+ * graphs packed into one device arena; BFS (source 0), CC and MIS run for all
+ * of them in one persistent launch per algorithm, one workgroup per graph,
+ * largest graphs first.  Outputs are int32[total_nodes] arrays; graph i's
+ * values start at its node offset. */
+typedef struct igx_batch {
+  int graphs;
+  long long total_nodes;
+  long long total_edges;
+  const long long* node_off;  /* [graphs + 1] */
+  const long long* edge_off;  /* [graphs + 1] */
+  const int* nindex;          /* graph i's rows start at node_off[i] + i (graph-local offsets) */
+  const int* nlist;           /* graph i's list starts at edge_off[i] (graph-local ids) */
+  const int* order;           /* processing order (largest first) */
+} igx_batch;
+
+int igx_batch_upload(int graphs, const igx_graph* host_graphs, igx_batch* dev);
+int igx_batch_free(igx_batch* dev);
+/* device igx_graph view of graph i inside the arena (for per-graph runs) */
+int igx_batch_graph_view(const igx_batch* b, int i, igx_graph* view);
+/* algo_mask: 1 BFS, 2 CC, 4 MIS.  block: 256, 512 or 1024 threads per
+ * workgroup.  use_graph != 0 captures the launches once as a hipGraph and
+ * replays it (re-captured when the batch, mask, block or outputs change). */
+int igx_batch_run(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int* d_mis, int block, int use_graph,
+                  igx_run_info* info);
+
+/* ------------------------------------------------------ selector network ---- */
+/* No reference counterpart.  MLP in -> 64 -> 64 -> out (ReLU, FP32) trained
+ * and evaluated with hand-written HIP kernels.  Parameters are laid out as
+ * W1[in][64], b1[64], W2[64][64], b2[64], W3[64][out], b3[out]. */
+int igx_mlp_create(int in, int out, uint64_t seed, void** handle);
+int igx_mlp_destroy(void* handle);
+int igx_mlp_param_count(void* handle);
+int igx_mlp_get_params(void* handle, float* host, size_t count);
+int igx_mlp_set_params(void* handle, const float* host, size_t count);
+/* host x[batch][in] -> host y[batch][out] */
+int igx_mlp_forward(void* handle, const float* host_x, int batch, float* host_y);
+/* gradient of sum(mask * (y - t)^2) / norm with respect to the parameters */
+int igx_mlp_gradients(void* handle, const float* x, const float* t, const float* mask, int batch, float norm,
+                      float* host_grads, float* loss);
+/* Adam over `epochs` passes of seeded-shuffled minibatches; *last_loss is the
+ * last epoch's mean masked loss */
+int igx_mlp_train(void* handle, const float* x, const float* t, const float* mask, int n, int epochs, int batch,
+                  float lr, uint64_t seed, float* last_loss);
+
 #ifdef __cplusplus
 }
 #endif

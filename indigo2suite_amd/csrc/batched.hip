@@ -1,0 +1,519 @@
+// Batched mode: BFS, CC and MIS over hundreds of uneven graphs packed into one
+// arena, one persistent launch per algorithm, optionally replayed as one
+// captured hipGraph.  (No reference counterpart: the reference runs one
+// program per graph.)
+//
+// Arena layout (device memory):
+//   node_off[G+1], edge_off[G+1]  int64 prefix offsets of the graphs
+//   nindex  int32[sum(n_i + 1)]   graph-local row offsets; graph i's rows
+//                                 start at node_off[i] + i
+//   nlist   int32[sum(m_i)]       graph-local neighbour ids; graph i's list
+//                                 starts at edge_off[i]
+//   outputs int32[sum(n_i)]       graph i's values start at node_off[i]
+//
+// Execution: a persistent grid; every workgroup repeatedly takes the next
+// graph from an atomic counter (graphs are ordered largest first, so the long
+// ones start early) and runs the whole algorithm on it with workgroup
+// barriers between rounds.  No workgroup ever waits for another, so every
+// wave reaches the loop exit once the counter passes G.
+//
+// Per-graph algorithms (results identical to the single-graph styles):
+//   BFS from vertex 0: level-synchronous top-down; each level expands the
+//     frontier queue with a workgroup-wide merge path over the degrees of a
+//     tile of kTile queue entries (tile = workgroup size), CAS-claimed
+//     discoveries pushed through an LDS counter.
+//   CC: union-find (larger root hooked under smaller with a CAS), rows walked
+//     with the same workgroup merge path, then full compression.
+//   MIS: pull rounds over a compacted worklist of undecided vertices with the
+//     unsigned hash priorities.
+#include "igx_internal.hpp"
+
+namespace igx {
+namespace {
+
+struct Arena {
+  const long long* node_off;
+  const long long* edge_off;
+  const int* nindex;
+  const int* nlist;
+  int graphs;
+};
+
+__device__ __forceinline__ void graph_view(const Arena& A, int g, int& n, const int*& nidx, const int*& nl,
+                                           long long& voff)
+{
+  voff = A.node_off[g];
+  n = (int)(A.node_off[g + 1] - voff);
+  nidx = A.nindex + voff + g;
+  nl = A.nlist + A.edge_off[g];
+}
+
+// exclusive scan over the workgroup; returns this thread's prefix, total via *tot
+template <int B>
+__device__ __forceinline__ int wg_excl_scan(int v, int* lds_w, int* tot)
+{
+  constexpr int W = B / kWave;
+  const int w = threadIdx.x / kWave;
+  const int incl = wave_incl_scan(v);
+  if (lane_id() == kWave - 1) lds_w[w] = incl;
+  __syncthreads();
+  int off = 0, t = 0;
+  #pragma unroll
+  for (int k = 0; k < W; k++) {
+    const int x = lds_w[k];
+    if (k < w) off += x;
+    t += x;
+  }
+  __syncthreads();
+  *tot = t;
+  return off + incl - v;
+}
+
+// Walk all edges of `items` rows given by row(k) for k in [0, count), tile by
+// tile: the workgroup scans the tile's degrees and every thread takes one
+// edge per step.  visit(row, e) for each edge.  All threads must call.
+template <int B, typename RowOf, typename F>
+__device__ __forceinline__ void wg_expand(int count, const int* __restrict__ nidx, RowOf row_of, int* s_excl,
+                                          int* s_row, int* s_beg, int* s_w, F visit)
+{
+  for (int base = 0; base < count; base += B) {
+    const int k = base + (int)threadIdx.x;
+    int v = 0, beg = 0, len = 0;
+    if (k < count) {
+      v = row_of(k);
+      beg = nidx[v];
+      len = nidx[v + 1] - beg;
+    }
+    int total = 0;
+    const int ex = wg_excl_scan<B>(len, s_w, &total);
+    s_excl[threadIdx.x] = ex;
+    s_row[threadIdx.x] = v;
+    s_beg[threadIdx.x] = beg;
+    __syncthreads();
+    for (int tb = 0; tb < total; tb += B) {
+      const int t = tb + (int)threadIdx.x;
+      if (t < total) {
+        int lo = 0, hi = B;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_excl[mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        visit(s_row[lo], s_beg[lo] + (t - s_excl[lo]));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int next_graph(int* counter, int* s_g)
+{
+  if (threadIdx.x == 0) *s_g = atomicAdd(counter, 1);
+  __syncthreads();
+  const int k = *s_g;
+  __syncthreads();
+  return k;
+}
+
+// ------------------------------------------------------------------ BFS ----
+template <int B>
+__global__ void __launch_bounds__(B) batched_bfs(Arena A, const int* __restrict__ order, int* counter,
+                                                  int* __restrict__ dist_all, int* __restrict__ queue_all)
+{
+  __shared__ int s_excl[B], s_row[B], s_beg[B];
+  __shared__ int s_w[B / kWave];
+  __shared__ int s_g, s_next;
+  while (true) {
+    const int k = next_graph(counter, &s_g);
+    if (k >= A.graphs) break;
+    const int g = order[k];
+    int n;
+    const int* nidx;
+    const int* nl;
+    long long voff;
+    graph_view(A, g, n, nidx, nl, voff);
+    int* dist = dist_all + voff;
+    int* q0 = queue_all + 2 * voff;
+    int* q1 = q0 + n;
+    for (int v = threadIdx.x; v < n; v += B) dist[v] = (v == 0) ? 0 : kInf;
+    if (threadIdx.x == 0) q0[0] = 0;
+    __syncthreads();
+    int qsize = 1;
+    int level = 0;
+    while (qsize > 0) {
+      if (threadIdx.x == 0) s_next = 0;
+      __syncthreads();
+      const int* qin = (level & 1) ? q1 : q0;
+      int* qout = (level & 1) ? q0 : q1;
+      const int nd = level + 1;
+      wg_expand<B>(qsize, nidx, [&](int i) { return qin[i]; }, s_excl, s_row, s_beg, s_w, [&](int, int e) {
+        const int u = nl[e];
+        bool disc = false;
+        if (dist[u] == kInf) {
+          int expected = kInf;
+          disc = __hip_atomic_compare_exchange_strong(&dist[u], &expected, nd, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        const uint64_t mask = ballot(disc);
+        if (mask != 0) {
+          const int leader = __ffsll((unsigned long long)mask) - 1;
+          int base = 0;
+          if (lane_id() == leader) base = atomicAdd(&s_next, __popcll(mask));
+          base = __shfl(base, leader, kWave);
+          if (disc) qout[base + mbcnt(mask)] = u;
+        }
+      });
+      __syncthreads();
+      qsize = s_next;
+      level++;
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------- CC ----
+__device__ __forceinline__ int wg_find(int* __restrict__ parent, int x)
+{
+  while (true) {
+    const int p = parent[x];
+    if (p == x) return x;
+    const int gp = parent[p];
+    if (gp == p) return p;
+    parent[x] = gp;
+    x = gp;
+  }
+}
+
+__device__ __forceinline__ void wg_unite(int* __restrict__ parent, int a, int b)
+{
+  int ra = wg_find(parent, a), rb = wg_find(parent, b);
+  while (ra != rb) {
+    const int hi = max(ra, rb), lo = min(ra, rb);
+    int expected = hi;
+    if (__hip_atomic_compare_exchange_strong(&parent[hi], &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP)) {
+      return;
+    }
+    ra = wg_find(parent, expected);
+    rb = wg_find(parent, lo);
+  }
+}
+
+template <int B>
+__global__ void __launch_bounds__(B) batched_cc(Arena A, const int* __restrict__ order, int* counter,
+                                                 int* __restrict__ label_all)
+{
+  __shared__ int s_excl[B], s_row[B], s_beg[B];
+  __shared__ int s_w[B / kWave];
+  __shared__ int s_g;
+  while (true) {
+    const int k = next_graph(counter, &s_g);
+    if (k >= A.graphs) break;
+    const int g = order[k];
+    int n;
+    const int* nidx;
+    const int* nl;
+    long long voff;
+    graph_view(A, g, n, nidx, nl, voff);
+    int* parent = label_all + voff;
+    for (int v = threadIdx.x; v < n; v += B) {
+      const int u0 = (nidx[v + 1] > nidx[v]) ? nl[nidx[v]] : v;
+      parent[v] = (u0 < v) ? u0 : v;
+    }
+    __syncthreads();
+    wg_expand<B>(n, nidx, [&](int i) { return i; }, s_excl, s_row, s_beg, s_w, [&](int v, int e) {
+      const int u = nl[e];
+      if (u < v) wg_unite(parent, v, u);
+    });
+    __syncthreads();
+    // read-only walks; each thread writes only its own vertex (see uf_compress)
+    for (int v = threadIdx.x; v < n; v += B) {
+      int x = parent[v];
+      int p = parent[x];
+      while (p != x) {
+        x = p;
+        p = parent[x];
+      }
+      parent[v] = x;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ MIS ----
+__device__ __forceinline__ bool mis_outranks(uint32_t pu, int u, uint32_t pv, int v)
+{
+  return (pu > pv) || (pu == pv && u > v);
+}
+
+template <int B>
+__global__ void __launch_bounds__(B) batched_mis(Arena A, const int* __restrict__ order, int* counter,
+                                                  int* __restrict__ status_all, int* __restrict__ wl_all)
+{
+  __shared__ int s_g, s_next;
+  while (true) {
+    const int k = next_graph(counter, &s_g);
+    if (k >= A.graphs) break;
+    const int g = order[k];
+    int n;
+    const int* nidx;
+    const int* nl;
+    long long voff;
+    graph_view(A, g, n, nidx, nl, voff);
+    int* status = status_all + voff;
+    int* w0 = wl_all + 2 * voff;
+    int* w1 = w0 + n;
+    for (int v = threadIdx.x; v < n; v += B) {
+      status[v] = 0;
+      w0[v] = v;
+    }
+    __syncthreads();
+    int count = n;
+    int round = 0;
+    while (count > 0) {
+      if (threadIdx.x == 0) s_next = 0;
+      __syncthreads();
+      const int* win = (round & 1) ? w1 : w0;
+      int* wout = (round & 1) ? w0 : w1;
+      for (int base = 0; base < count; base += B) {
+        const int idx = base + (int)threadIdx.x;
+        bool keep = false;
+        int v = 0;
+        if (idx < count) {
+          v = win[idx];
+          const uint32_t pv = mis_priority(v);
+          int decision = 1;  // included unless blocked
+          for (int i = nidx[v]; i < nidx[v + 1]; i++) {
+            const int u = nl[i];
+            if (u == v) continue;
+            const int su = __hip_atomic_load(&status[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (su == 1) {
+              decision = 2;
+              break;
+            }
+            if (su != 2 && mis_outranks(mis_priority(u), u, pv, v)) decision = 0;
+          }
+          if (decision == 0) keep = true;
+          else __hip_atomic_store(&status[v], decision, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        const uint64_t mask = ballot(keep);
+        if (mask != 0) {
+          const int leader = __ffsll((unsigned long long)mask) - 1;
+          int b0 = 0;
+          if (lane_id() == leader) b0 = atomicAdd(&s_next, __popcll(mask));
+          b0 = __shfl(b0, leader, kWave);
+          if (keep) wout[b0 + mbcnt(mask)] = v;
+        }
+      }
+      __syncthreads();
+      count = s_next;
+      round++;
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void batched_reset(int* counters, int count)
+{
+  if ((int)threadIdx.x < count) counters[threadIdx.x] = 0;
+}
+
+// ------------------------------------------------------------ host side ----
+struct BatchState {
+  igx_batch key = {};
+  int block = 0;
+  int mask = 0;
+  int* outs[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+BatchState g_state;
+
+template <int B>
+void launch_all(const igx_batch* b, const Arena& A, int mask, int* const* outs, int* counters, int* scratch, int grid,
+                hipStream_t s)
+{
+  hipLaunchKernelGGL(batched_reset, dim3(1), dim3(64), 0, s, counters, 3);
+  if (mask & 1) hipLaunchKernelGGL((batched_bfs<B>), dim3(grid), dim3(B), 0, s, A, b->order, counters + 0, outs[0],
+                                   scratch);
+  if (mask & 2) hipLaunchKernelGGL((batched_cc<B>), dim3(grid), dim3(B), 0, s, A, b->order, counters + 1, outs[1]);
+  if (mask & 4) hipLaunchKernelGGL((batched_mis<B>), dim3(grid), dim3(B), 0, s, A, b->order, counters + 2, outs[2],
+                                   scratch + 2 * b->total_nodes);
+}
+
+int enqueue(const igx_batch* b, int mask, int* const* outs, int block, hipStream_t s)
+{
+  Runtime& R = rt();
+  void* pc = nullptr;
+  void* ps = nullptr;
+  IGX_TRY(ws_get(WS_SCRATCH, sizeof(int) * 64, &pc));
+  IGX_TRY(ws_get(WS_H, sizeof(int) * (size_t)(4 * b->total_nodes + 64), &ps));
+  Arena A{b->node_off, b->edge_off, b->nindex, b->nlist, b->graphs};
+  const int per_cu = R.max_threads_per_cu / block;
+  int grid = R.cus * (per_cu > 0 ? per_cu : 1);
+  if (grid > b->graphs) grid = b->graphs;
+  if (grid < 1) grid = 1;
+  switch (block) {
+    case 256: launch_all<256>(b, A, mask, outs, (int*)pc, (int*)ps, grid, s); break;
+    case 512: launch_all<512>(b, A, mask, outs, (int*)pc, (int*)ps, grid, s); break;
+    case 1024: launch_all<1024>(b, A, mask, outs, (int*)pc, (int*)ps, grid, s); break;
+    default: return IGX_ERR_ARG;
+  }
+  IGX_HIP(hipGetLastError());
+  return IGX_OK;
+}
+
+bool same_batch(const igx_batch& a, const igx_batch& b)
+{
+  return a.graphs == b.graphs && a.total_nodes == b.total_nodes && a.total_edges == b.total_edges &&
+         a.node_off == b.node_off && a.edge_off == b.edge_off && a.nindex == b.nindex && a.nlist == b.nlist &&
+         a.order == b.order;
+}
+
+void drop_graph()
+{
+  if (g_state.exec) hipGraphExecDestroy(g_state.exec);
+  if (g_state.graph) hipGraphDestroy(g_state.graph);
+  g_state.exec = nullptr;
+  g_state.graph = nullptr;
+}
+
+}  // namespace
+}  // namespace igx
+
+using namespace igx;
+
+extern "C" {
+
+int igx_batch_upload(int graphs, const igx_graph* host, igx_batch* out)
+{
+  IGX_TRY(ensure_init());
+  if (graphs < 1 || host == nullptr || out == nullptr) return IGX_ERR_ARG;
+  std::vector<long long> noff(graphs + 1, 0), eoff(graphs + 1, 0);
+  for (int i = 0; i < graphs; i++) {
+    if (host[i].nodes < 1 || host[i].edges < 0 || host[i].nindex == nullptr) return IGX_ERR_ARG;
+    const int rc = igx_check_csr(&host[i]);
+    if (rc != IGX_OK) return rc;
+    noff[i + 1] = noff[i] + host[i].nodes;
+    eoff[i + 1] = eoff[i] + host[i].edges;
+  }
+  // largest graphs first so the longest workgroups start early
+  std::vector<int> order(graphs);
+  for (int i = 0; i < graphs; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (long long)host[a].edges + host[a].nodes > (long long)host[b].edges + host[b].nodes;
+  });
+  igx_batch b = {};
+  b.graphs = graphs;
+  b.total_nodes = noff[graphs];
+  b.total_edges = eoff[graphs];
+  void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr, *p5 = nullptr;
+  if (igx_alloc(&p1, sizeof(long long) * (graphs + 1)) != IGX_OK ||
+      igx_alloc(&p2, sizeof(long long) * (graphs + 1)) != IGX_OK ||
+      igx_alloc(&p3, sizeof(int) * (size_t)(b.total_nodes + graphs)) != IGX_OK ||
+      igx_alloc(&p4, sizeof(int) * (size_t)(b.total_edges > 0 ? b.total_edges : 1)) != IGX_OK ||
+      igx_alloc(&p5, sizeof(int) * (size_t)graphs) != IGX_OK) {
+    igx_free(p1); igx_free(p2); igx_free(p3); igx_free(p4); igx_free(p5);
+    return IGX_ERR_NOMEM;
+  }
+  IGX_TRY(igx_memcpy_h2d(p1, noff.data(), sizeof(long long) * (graphs + 1)));
+  IGX_TRY(igx_memcpy_h2d(p2, eoff.data(), sizeof(long long) * (graphs + 1)));
+  IGX_TRY(igx_memcpy_h2d(p5, order.data(), sizeof(int) * graphs));
+  for (int i = 0; i < graphs; i++) {
+    IGX_TRY(igx_memcpy_h2d((int*)p3 + noff[i] + i, host[i].nindex, sizeof(int) * ((size_t)host[i].nodes + 1)));
+    if (host[i].edges > 0) {
+      IGX_TRY(igx_memcpy_h2d((int*)p4 + eoff[i], host[i].nlist, sizeof(int) * (size_t)host[i].edges));
+    }
+  }
+  b.node_off = (const long long*)p1;
+  b.edge_off = (const long long*)p2;
+  b.nindex = (const int*)p3;
+  b.nlist = (const int*)p4;
+  b.order = (const int*)p5;
+  *out = b;
+  return IGX_OK;
+}
+
+int igx_batch_free(igx_batch* b)
+{
+  if (b == nullptr) return IGX_ERR_ARG;
+  if (same_batch(g_state.key, *b)) {
+    drop_graph();
+    g_state.key = igx_batch{};
+  }
+  igx_free((void*)b->node_off);
+  igx_free((void*)b->edge_off);
+  igx_free((void*)b->nindex);
+  igx_free((void*)b->nlist);
+  igx_free((void*)b->order);
+  *b = igx_batch{};
+  return IGX_OK;
+}
+
+int igx_batch_graph_view(const igx_batch* b, int i, igx_graph* view)
+{
+  if (b == nullptr || view == nullptr || i < 0 || i >= b->graphs) return IGX_ERR_ARG;
+  long long no[2], eo[2];
+  IGX_TRY(igx_memcpy_d2h(no, b->node_off + i, sizeof(no)));
+  IGX_TRY(igx_memcpy_d2h(eo, b->edge_off + i, sizeof(eo)));
+  view->nodes = (int)(no[1] - no[0]);
+  view->edges = (int)(eo[1] - eo[0]);
+  view->nindex = const_cast<int*>(b->nindex) + no[0] + i;
+  view->nlist = const_cast<int*>(b->nlist) + eo[0];
+  view->eweight = nullptr;
+  return IGX_OK;
+}
+
+int igx_batch_run(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int* d_mis, int block, int use_graph,
+                  igx_run_info* info)
+{
+  IGX_TRY(ensure_init());
+  Runtime& R = rt();
+  if (b == nullptr || b->graphs < 1 || (algo_mask & ~7) != 0 || algo_mask == 0) return IGX_ERR_ARG;
+  if (((algo_mask & 1) && !d_bfs) || ((algo_mask & 2) && !d_cc) || ((algo_mask & 4) && !d_mis)) return IGX_ERR_ARG;
+  if (block != 256 && block != 512 && block != 1024) return IGX_ERR_ARG;
+  int* outs[3] = {d_bfs, d_cc, d_mis};
+  igx_run_info local = {};
+  if (!use_graph) {
+    IGX_TRY(timer_start());
+    IGX_TRY(enqueue(b, algo_mask, outs, block, R.stream));
+    IGX_TRY(timer_stop(&local.seconds));
+    local.launches = 1 + ((algo_mask & 1) != 0) + ((algo_mask & 2) != 0) + ((algo_mask & 4) != 0);
+  } else {
+    const bool same = same_batch(g_state.key, *b) && g_state.block == block &&
+                      g_state.mask == algo_mask && g_state.outs[0] == d_bfs && g_state.outs[1] == d_cc &&
+                      g_state.outs[2] == d_mis && g_state.exec != nullptr;
+    if (!same) {
+      drop_graph();
+      // workspaces must exist before capture (no allocation inside it)
+      void* tmp = nullptr;
+      IGX_TRY(ws_get(WS_SCRATCH, sizeof(int) * 64, &tmp));
+      IGX_TRY(ws_get(WS_H, sizeof(int) * (size_t)(4 * b->total_nodes + 64), &tmp));
+      IGX_HIP(hipStreamSynchronize(R.stream));
+      IGX_HIP(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue(b, algo_mask, outs, block, R.stream);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(R.stream, &graph);
+      if (rc != IGX_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+      }
+      IGX_HIP(e);
+      g_state.graph = graph;
+      IGX_HIP(hipGraphInstantiate(&g_state.exec, graph, nullptr, nullptr, 0));
+      g_state.key = *b;
+      g_state.block = block;
+      g_state.mask = algo_mask;
+      for (int k = 0; k < 3; k++) g_state.outs[k] = outs[k];
+    }
+    IGX_TRY(timer_start());
+    IGX_HIP(hipGraphLaunch(g_state.exec, R.stream));
+    IGX_TRY(timer_stop(&local.seconds));
+    local.launches = 1;
+  }
+  local.iterations = b->graphs;
+  if (info) *info = local;
+  return IGX_OK;
+}
+
+}  // extern "C"

@@ -222,7 +222,7 @@ __device__ __forceinline__ void uf_unite(int* __restrict__ parent, int a, int b)
   }
 }
 
-// parent[v] = first smaller neighbour (or v); vertices with more than kChunk
+// parent[v] = the first neighbour if it is smaller (else v); vertices with more than kChunk
 // edges also get their edge chunks queued for the balanced hook pass.
 __global__ void __launch_bounds__(kBlock) uf_init(const int* __restrict__ nindex, const int* __restrict__ nlist,
                                                    int n, int* __restrict__ parent, Entry* __restrict__ bigq,
@@ -235,15 +235,10 @@ __global__ void __launch_bounds__(kBlock) uf_init(const int* __restrict__ nindex
     if (v < n) {
       beg = nindex[v];
       end = nindex[v + 1];
-      int pv = v;
-      for (int i = beg; i < end; i++) {
-        const int u = nlist[i];
-        if (u < v) {
-          pv = u;
-          break;
-        }
-      }
-      parent[v] = pv;
+      // any smaller neighbour is a valid first parent; with sorted adjacency
+      // the first neighbour is the smallest one
+      const int u0 = (end > beg) ? nlist[beg] : v;
+      parent[v] = (u0 < v) ? u0 : v;
     }
     push_vertex_chunks(bigq, bigq_count, (end - beg) > kChunk, v, beg, end);
   }
@@ -263,8 +258,10 @@ __global__ void __launch_bounds__(kBlock) uf_hook(const int* __restrict__ nindex
   const int lane = lane_id();
   const int gw = blockIdx.x * kWavesPerBlock + w;
   const int nwaves = gridDim.x * kWavesPerBlock;
+  __shared__ int s_root[kWavesPerBlock][kWave];
   int* excl = s_excl[w];
   int* rbeg = s_beg[w];
+  int* rroot = s_root[w];
   for (int v0 = gw * kWave; v0 < n; v0 += nwaves * kWave) {
     const int v = v0 + lane;
     int beg = 0, len = 0;
@@ -277,20 +274,59 @@ __global__ void __launch_bounds__(kBlock) uf_hook(const int* __restrict__ nindex
     const int total = wave_bcast(incl, kWave - 1);
     excl[lane] = incl - len;
     rbeg[lane] = beg;
+    // the row's current root, reused as a hint by every edge of the row
+    rroot[lane] = (len > 0) ? uf_find(parent, v) : v;
     __builtin_amdgcn_wave_barrier();
-    for (int tb = 0; tb < total; tb += kWave) {
-      const int t = tb + lane;
-      if (t < total) {
-        int lo = 0, hi = kWave;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (excl[mid] <= t) lo = mid;
-          else hi = mid;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // kIlp independent edges per lane per step so their loads overlap
+    constexpr int kIlp = 4;
+    for (int tb = 0; tb < total; tb += kIlp * kWave) {
+      int u[kIlp], row[kIlp], hint[kIlp], pu[kIlp], ph[kIlp], ppu[kIlp];
+      bool need[kIlp];
+      #pragma unroll
+      for (int k = 0; k < kIlp; k++) {
+        const int t = tb + k * kWave + lane;
+        need[k] = false;
+        row[k] = 0;
+        hint[k] = 0;
+        u[k] = 0;
+        if (t < total) {
+          int lo = 0, hi = kWave;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (excl[mid] <= t) lo = mid;
+            else hi = mid;
+          }
+          row[k] = v0 + lo;
+          hint[k] = rroot[lo];
+          u[k] = nlist[rbeg[lo] + (t - excl[lo])];
+          need[k] = true;
         }
-        const int row = v0 + lo;
-        const int e = rbeg[lo] + (t - excl[lo]);
-        const int u = nlist[e];
-        if (u < row) uf_unite(parent, row, u);
+      }
+      #pragma unroll
+      for (int k = 0; k < kIlp; k++) {
+        need[k] = need[k] && u[k] < row[k];
+        pu[k] = need[k] ? parent[u[k]] : 0;
+        ph[k] = need[k] ? parent[hint[k]] : 0;
+      }
+      #pragma unroll
+      for (int k = 0; k < kIlp; k++) ppu[k] = (need[k] && pu[k] != u[k]) ? parent[pu[k]] : pu[k];
+      #pragma unroll
+      for (int k = 0; k < kIlp; k++) {
+        if (!need[k]) continue;
+        // two levels are resolved above; deeper trees fall back to the loop
+        int ru = (pu[k] == u[k]) ? u[k] : ((ppu[k] == pu[k]) ? pu[k] : uf_find(parent, ppu[k]));
+        int ra = (ph[k] == hint[k]) ? hint[k] : uf_find(parent, ph[k]);
+        while (ra != ru) {
+          const int hi2 = max(ra, ru), lo2 = min(ra, ru);
+          int expected = hi2;
+          if (__hip_atomic_compare_exchange_strong(&parent[hi2], &expected, lo2, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            break;
+          }
+          ra = uf_find(parent, expected);
+          ru = uf_find(parent, lo2);
+        }
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -302,10 +338,22 @@ __global__ void __launch_bounds__(kBlock) uf_hook(const int* __restrict__ nindex
   });
 }
 
+// Final labels.  The walk is read-only and each thread writes only its own
+// vertex: a concurrent path-halving write could otherwise replace another
+// vertex's final root with a non-root ancestor.  Every value read is the
+// vertex's old ancestor or its root, so each walk ends at the root.
 __global__ void __launch_bounds__(kBlock) uf_compress(int n, int* __restrict__ parent)
 {
   const int stride = gridDim.x * kBlock;
-  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) parent[v] = uf_find(parent, v);
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    int x = parent[v];
+    int p = parent[x];
+    while (p != x) {
+      x = p;
+      p = parent[x];
+    }
+    parent[v] = x;
+  }
 }
 
 // ------------------------------------------------------------ host side ----

@@ -1,0 +1,177 @@
+"""Learned style selector: an MLP (16 -> 64 -> 64 -> K, ReLU, FP32) over cheap
+graph statistics that predicts, for each of an algorithm's K registered
+styles, its slowdown against the fastest style; the pick is the style with
+the smallest prediction.  Training and inference run on the GPU through the
+hand-written HIP kernels of libindigo_hip.so (igx_mlp_*); the exhaustive
+autotuner (autotune.sweep) is both the fallback and the judge of regret.
+
+Targets: y_k = log(t_k / t_best), clipped at log(kClip); styles without a
+measurement are masked out of the loss.  Features are standardised with the
+training set's mean and standard deviation, which are saved with the weights.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional
+
+import numpy as np
+
+from . import _native
+from ._native import check
+
+kClip = 100.0
+
+
+def _lib():
+    lib = _native.hip_lib()
+    if not hasattr(lib, "_mlp_sigs"):
+        vp = ctypes.c_void_p
+        fp = ctypes.POINTER(ctypes.c_float)
+        lib.igx_mlp_create.restype = ctypes.c_int
+        lib.igx_mlp_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]
+        lib.igx_mlp_destroy.restype = ctypes.c_int
+        lib.igx_mlp_destroy.argtypes = [vp]
+        lib.igx_mlp_param_count.restype = ctypes.c_int
+        lib.igx_mlp_param_count.argtypes = [vp]
+        lib.igx_mlp_get_params.restype = ctypes.c_int
+        lib.igx_mlp_get_params.argtypes = [vp, fp, ctypes.c_size_t]
+        lib.igx_mlp_set_params.restype = ctypes.c_int
+        lib.igx_mlp_set_params.argtypes = [vp, fp, ctypes.c_size_t]
+        lib.igx_mlp_forward.restype = ctypes.c_int
+        lib.igx_mlp_forward.argtypes = [vp, fp, ctypes.c_int, fp]
+        lib.igx_mlp_gradients.restype = ctypes.c_int
+        lib.igx_mlp_gradients.argtypes = [vp, fp, fp, fp, ctypes.c_int, ctypes.c_float, fp, fp]
+        lib.igx_mlp_train.restype = ctypes.c_int
+        lib.igx_mlp_train.argtypes = [vp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                      ctypes.c_uint64, fp]
+        lib._mlp_sigs = True
+    return lib
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """Device MLP in -> 64 -> 64 -> out (libindigo_hip.so igx_mlp_*)."""
+
+    HIDDEN = 64
+
+    def __init__(self, n_in: int, n_out: int, seed: int = 1):
+        self.n_in, self.n_out = n_in, n_out
+        self.h = ctypes.c_void_p()
+        check(_lib().igx_mlp_create(n_in, n_out, seed, ctypes.byref(self.h)), "igx_mlp_create")
+        self.nparams = _lib().igx_mlp_param_count(self.h)
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.nparams, np.float32)
+        check(_lib().igx_mlp_get_params(self.h, _fp(p), self.nparams), "igx_mlp_get_params")
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        check(_lib().igx_mlp_set_params(self.h, _fp(p), p.size), "igx_mlp_set_params")
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((x.shape[0], self.n_out), np.float32)
+        check(_lib().igx_mlp_forward(self.h, _fp(x), x.shape[0], _fp(y)), "igx_mlp_forward")
+        return y
+
+    def gradients(self, x, t, mask, norm: Optional[float] = None):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        mask = np.ascontiguousarray(mask, np.float32)
+        norm = float(mask.sum()) if norm is None else norm
+        g = np.empty(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        check(_lib().igx_mlp_gradients(self.h, _fp(x), _fp(t), _fp(mask), x.shape[0], norm, _fp(g),
+                                       ctypes.byref(loss)), "igx_mlp_gradients")
+        return g, loss.value
+
+    def train(self, x, t, mask, epochs: int, batch: int = 64, lr: float = 1e-3, seed: int = 1) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        mask = np.ascontiguousarray(mask, np.float32)
+        loss = ctypes.c_float()
+        check(_lib().igx_mlp_train(self.h, _fp(x), _fp(t), _fp(mask), x.shape[0], epochs, batch, lr, seed,
+                                   ctypes.byref(loss)), "igx_mlp_train")
+        return loss.value
+
+    def free(self):
+        if self.h:
+            _lib().igx_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+
+def unpack(p: np.ndarray, n_in: int, n_out: int):
+    """Split a flat parameter vector into (W1, b1, W2, b2, W3, b3)."""
+    H = Mlp.HIDDEN
+    sizes = [n_in * H, H, H * H, H, H * n_out, n_out]
+    parts = np.split(p.astype(np.float64), np.cumsum(sizes)[:-1])
+    W1, b1, W2, b2, W3, b3 = parts
+    return W1.reshape(n_in, H), b1, W2.reshape(H, H), b2, W3.reshape(H, n_out), b3
+
+
+def targets_from_times(times: np.ndarray):
+    """(y, mask): y = log(t / t_best) clipped at log(kClip); NaN/inf times masked."""
+    t = np.asarray(times, np.float64)
+    ok = np.isfinite(t) & (t > 0)
+    best = np.where(ok, t, np.inf).min(axis=1, keepdims=True)
+    y = np.where(ok, np.log(np.minimum(t / best, kClip)), 0.0)
+    return y.astype(np.float32), ok.astype(np.float32)
+
+
+class Selector:
+    def __init__(self, algo: str, styles: List[str], seed: int = 1):
+        self.algo = algo
+        self.styles = list(styles)
+        self.mean = None
+        self.std = None
+        self.net: Optional[Mlp] = None
+        self.seed = seed
+
+    def _norm(self, f: np.ndarray) -> np.ndarray:
+        return ((np.asarray(f, np.float64) - self.mean) / self.std).astype(np.float32)
+
+    def fit(self, features: np.ndarray, times: np.ndarray, epochs: int = 200, batch: int = 64,
+            lr: float = 2e-3) -> float:
+        f = np.asarray(features, np.float64)
+        self.mean = f.mean(axis=0)
+        self.std = np.where(f.std(axis=0) > 1e-12, f.std(axis=0), 1.0)
+        y, m = targets_from_times(times)
+        if self.net is None:
+            self.net = Mlp(f.shape[1], len(self.styles), seed=self.seed)
+        return self.net.train(self._norm(f), y, m, epochs=epochs, batch=batch, lr=lr, seed=self.seed)
+
+    def predict_scores(self, features: np.ndarray) -> np.ndarray:
+        return self.net.forward(self._norm(np.atleast_2d(features)))
+
+    def pick(self, features: np.ndarray) -> List[int]:
+        return list(np.argmin(self.predict_scores(features), axis=1))
+
+    def evaluate(self, features: np.ndarray, times: np.ndarray, default_style: int) -> dict:
+        t = np.asarray(times, np.float64)
+        picks = self.pick(features)
+        best = np.nanmin(np.where(np.isfinite(t), t, np.nan), axis=1)
+        chosen = t[np.arange(t.shape[0]), picks]
+        regret = chosen / best - 1.0
+        vs_default = t[:, default_style] / chosen
+        return {"mean_regret": float(np.mean(regret)), "median_regret": float(np.median(regret)),
+                "max_regret": float(np.max(regret)), "frac_within_10pct": float(np.mean(regret <= 0.10)),
+                "geomean_speedup_vs_default": float(np.exp(np.mean(np.log(vs_default)))),
+                "frac_faster_than_default": float(np.mean(vs_default > 1.0)),
+                "picks": [self.styles[p] for p in picks]}
+
+    def save(self, path: str) -> None:
+        np.savez_compressed(path, algo=self.algo, styles=np.array(self.styles), mean=self.mean, std=self.std,
+                            params=self.net.params(), n_in=self.net.n_in)
+
+    @classmethod
+    def load(cls, path: str) -> "Selector":
+        d = np.load(path)
+        s = cls(str(d["algo"]), [str(x) for x in d["styles"]])
+        s.mean, s.std = d["mean"], d["std"]
+        s.net = Mlp(int(d["n_in"]), len(s.styles))
+        s.net.set_params(d["params"])
+        return s

@@ -64,6 +64,9 @@ def test_driver_runs_programs_on_fixtures(tmp_path, gpu):
                              cwd=str(tmp_path), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                              timeout=300, env=dict(os.environ, PYTHONPATH=run_codes.REPO_DIR))
         text = out.stdout
+        with open(os.path.join(str(tmp_path), f"driver_{algo}.log"), "w") as f:
+            f.write(text)
+        assert "exit status" not in text, [ln for ln in text.splitlines() if "exit status" in ln]
         assert "ERROR" not in text, text[-3000:]
         runs = len(keep) * len(os.listdir(inputs))
         if algo in ("bfs", "sssp", "cc"):
