@@ -284,8 +284,14 @@ constexpr int kAlpha = 14;
 constexpr int kBeta = 24;
 
 __global__ void __launch_bounds__(kBlock) bfs_do_init(int src, const int* __restrict__ nindex, int m,
-                                                       Entry* __restrict__ q0, Ctl* ctl)
+                                                       Entry* __restrict__ q0, unsigned* __restrict__ bm, int words,
+                                                       Ctl* ctl)
 {
+  // bitmaps 0 and 1 start empty (2 is cleared during level 0); src is in 0
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < 2 * words; i += stride) {
+    bm[i] = (i == (src >> 5)) ? (1u << (src & 31)) : 0u;
+  }
   if (blockIdx.x != 0) return;
   const int beg = nindex[src], end = nindex[src + 1];
   const int c = chunks_of(end - beg);
@@ -318,10 +324,24 @@ __device__ __forceinline__ void block_add_ll(long long* target, long long v, lon
   __syncthreads();
 }
 
+__device__ __forceinline__ bool in_bits(const unsigned* __restrict__ bits, int w)
+{
+  return (bits[w >> 5] >> (w & 31)) & 1u;
+}
+
+// One BFS level.  Three frontier bitmaps rotate: bm[level % 3] holds the
+// current frontier, bm[(level + 1) % 3] receives the next one and
+// bm[(level + 2) % 3] (the previous frontier) is cleared for the level after.
+// Top-down: expand the chunk queue, claim undiscovered vertices with a CAS,
+// set their bits, queue their chunks.  Bottom-up: every unvisited vertex looks
+// for a neighbour in the current frontier bitmap -- its first four neighbours
+// with independent loads, the rest of a long row by the whole wave with an
+// early exit -- and the wave writes its 64 bits of the next bitmap at once.
 __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ nindex,
                                                         const int* __restrict__ nlist, int n,
                                                         int* __restrict__ dist, Entry* __restrict__ q0,
-                                                        Entry* __restrict__ q1, Ctl* ctl)
+                                                        Entry* __restrict__ q1, unsigned* __restrict__ bm, int words,
+                                                        Ctl* ctl)
 {
   __shared__ int s_excl[kWavesPerBlock][kWave];
   __shared__ Entry s_ent[kWavesPerBlock][kWave];
@@ -333,10 +353,18 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
   const Entry* q_in = p ? q1 : q0;
   Entry* q_out = p ? q0 : q1;
   int* out_count = &ctl->wl_count[p ^ 1];
+  unsigned* cur = bm + (size_t)(level % 3) * words;
+  unsigned* next = bm + (size_t)((level + 1) % 3) * words;
+  unsigned* stale = bm + (size_t)((level + 2) % 3) * words;
   const int nd = level + 1;
   const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
   const int gw = blockIdx.x * kWavesPerBlock + w;
   const int nwaves = gridDim.x * kWavesPerBlock;
+  {
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < words; i += stride) stale[i] = 0u;
+  }
   long long found_v = 0, found_e = 0;
   if (mode == 0) {
     const int qsize = ctl->wl_count[p];
@@ -349,6 +377,7 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
         disc = __hip_atomic_compare_exchange_strong(&dist[u], &expected, nd, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
         if (disc) {
+          atomicOr(&next[u >> 5], 1u << (u & 31));
           ub = nindex[u];
           ue = nindex[u + 1];
           found_v += 1;
@@ -358,26 +387,47 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
       push_vertex_chunks(q_out, out_count, disc, u, ub, ue);
     });
   } else {
+    constexpr int kFirst = 4;
     for (int base = gw * kWave; base < n; base += nwaves * kWave) {
-      const int u = base + lane_id();
-      bool disc = false;
+      const int u = base + lane;
+      bool unvisited = false, found = false;
       int ub = 0, ue = 0;
       if (u < n && dist[u] == kInf) {
+        unvisited = true;
         ub = nindex[u];
         ue = nindex[u + 1];
-        for (int i = ub; i < ue; i++) {
-          if (dist[nlist[i]] == level) {
-            disc = true;
+        int nb[kFirst];
+        #pragma unroll
+        for (int k = 0; k < kFirst; k++) nb[k] = (ub + k < ue) ? nlist[ub + k] : -1;
+        #pragma unroll
+        for (int k = 0; k < kFirst; k++) found = found || (nb[k] >= 0 && in_bits(cur, nb[k]));
+      }
+      uint64_t pending = ballot(unvisited && !found && ue - ub > kFirst);
+      while (pending != 0) {
+        const int L = __ffsll((unsigned long long)pending) - 1;
+        pending &= pending - 1;
+        const int s0 = __shfl(ub, L, kWave) + kFirst;
+        const int e0 = __shfl(ue, L, kWave);
+        bool hit = false;
+        for (int i0 = s0; i0 < e0; i0 += kWave) {
+          const int i = i0 + lane;
+          const bool h = (i < e0) && in_bits(cur, nlist[i]);
+          if (ballot(h) != 0) {
+            hit = true;
             break;
           }
         }
-        if (disc) {
-          dist[u] = nd;
-          found_v += 1;
-          found_e += ue - ub;
-        }
+        if (lane == L) found = hit;
       }
-      push_vertex_chunks(q_out, out_count, disc, u, ub, ue);
+      if (found) {
+        dist[u] = nd;
+        found_v += 1;
+        found_e += ue - ub;
+      }
+      const uint64_t mask = ballot(found);
+      if (lane == 0) next[base >> 5] = (unsigned)mask;
+      if (lane == 1) next[(base >> 5) + 1] = (unsigned)(mask >> 32);
+      push_vertex_chunks(q_out, out_count, found, u, ub, ue);
     }
   }
   block_add_ll(&ctl->aux_ll[0], found_v, s_red);
@@ -545,23 +595,25 @@ int style_do_hybrid(const igx_graph* g, int src, void* out, const void*, igx_run
   Runtime& R = rt();
   int* dist = (int*)out;
   const size_t cap = (size_t)g->nodes + (size_t)g->edges / kChunk + 64;
-  void* p0 = nullptr;
-  void* p1 = nullptr;
+  const int words = ((g->nodes + kWave - 1) / kWave) * 2;
+  void *p0 = nullptr, *p1 = nullptr, *pb = nullptr;
   IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &p0));
   IGX_TRY(ws_get(WS_E, sizeof(Entry) * cap, &p1));
+  IGX_TRY(ws_get(WS_F, sizeof(unsigned) * 3 * (size_t)words, &pb));
   IGX_TRY(timer_start());
   IGX_TRY(ctl_reset(g->nodes));
   hipLaunchKernelGGL(bfs_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
-  hipLaunchKernelGGL(bfs_do_init, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, g->edges, (Entry*)p0, R.d_ctl);
+  hipLaunchKernelGGL(bfs_do_init, dim3(blocks_for(2 * (long long)words, kBlock)), dim3(kBlock), 0, R.stream, src,
+                     g->nindex, g->edges, (Entry*)p0, (unsigned*)pb, words, R.d_ctl);
   int rounds = 0, launches = 0;
   const int rc = run_rounds(
       [&](long long) {
         ktimer_begin("bfs_do_level");
         hipLaunchKernelGGL(bfs_do_level, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
-                           g->nodes, dist, (Entry*)p0, (Entry*)p1, R.d_ctl);
+                           g->nodes, dist, (Entry*)p0, (Entry*)p1, (unsigned*)pb, words, R.d_ctl);
         ktimer_end("bfs_do_level");
       },
-      (long long)g->nodes + 2, &rounds, &launches);
+      (long long)g->nodes + 2, &rounds, &launches, 8);
   IGX_TRY(timer_stop(&info->seconds));
   info->iterations = rounds;
   info->launches = launches;

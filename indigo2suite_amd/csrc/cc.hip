@@ -356,6 +356,141 @@ __global__ void __launch_bounds__(kBlock) uf_compress(int n, int* __restrict__ p
   }
 }
 
+// -------------------------------------------------- sampled union-find ----
+// Link every vertex to its first kAfRounds neighbours, compress, find the
+// root shared by most of 1024 sampled vertices (the giant component), then
+// hook the remaining edges of only those vertices outside it.  An edge with
+// an endpoint outside the sampled component is processed from that endpoint;
+// an edge with both endpoints inside joins nothing new.  Skipping is decided
+// by reading parent[v] == c, which only ever holds for vertices already
+// connected to c.  The root of every tree is still its minimum id.
+constexpr int kAfRounds = 2;
+constexpr int kAfSamples = 1024;
+
+__global__ void __launch_bounds__(kBlock) af_init(const int* __restrict__ nindex, int n, int* __restrict__ parent,
+                                                   Entry* __restrict__ bigq, int* __restrict__ bigq_count)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < n; base += stride) {
+    const int v = base + threadIdx.x;
+    int beg = 0, end = 0;
+    if (v < n) {
+      beg = nindex[v];
+      end = nindex[v + 1];
+      parent[v] = v;
+    }
+    push_vertex_chunks(bigq, bigq_count, (end - beg) > kChunk, v, beg, end);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) af_link_first(const int* __restrict__ nindex,
+                                                         const int* __restrict__ nlist, int n,
+                                                         int* __restrict__ parent)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const int beg = nindex[v], end = nindex[v + 1];
+    #pragma unroll
+    for (int r = 0; r < kAfRounds; r++) {
+      if (beg + r < end) uf_unite(parent, v, nlist[beg + r]);
+    }
+  }
+}
+
+// one workgroup: the most frequent root among kAfSamples hashed vertices
+__global__ void __launch_bounds__(kBlock) af_sample(int n, const int* __restrict__ parent, int* __restrict__ out)
+{
+  __shared__ int s_key[kAfSamples];
+  for (int i = threadIdx.x; i < kAfSamples; i += kBlock) {
+    uint32_t h = (uint32_t)i * 2654435761u + 0x9E3779B9u;
+    h ^= h >> 16;
+    h *= 0x45d9f3bu;
+    h ^= h >> 16;
+    s_key[i] = parent[(int)(h % (uint32_t)n)];
+  }
+  __syncthreads();
+  // bitonic sort of the sampled roots
+  for (int k = 2; k <= kAfSamples; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < kAfSamples; i += kBlock) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int a = s_key[i], b = s_key[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            s_key[i] = b;
+            s_key[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    int best = s_key[0], best_run = 0, run = 0;
+    for (int i = 0; i < kAfSamples; i++) {
+      run = (i > 0 && s_key[i] == s_key[i - 1]) ? run + 1 : 1;
+      if (run > best_run) {
+        best_run = run;
+        best = s_key[i];
+      }
+    }
+    *out = best;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) af_link_rest(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                        int n, int* __restrict__ parent,
+                                                        const Entry* __restrict__ bigq,
+                                                        const int* __restrict__ bigq_count,
+                                                        const int* __restrict__ giant)
+{
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
+  __shared__ Entry s_ent[kWavesPerBlock][kWave];
+  const int c = *giant;
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int gw = blockIdx.x * kWavesPerBlock + w;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  int* excl = s_excl[w];
+  int* rbeg = s_beg[w];
+  for (int v0 = gw * kWave; v0 < n; v0 += nwaves * kWave) {
+    const int v = v0 + lane;
+    int beg = 0, len = 0;
+    if (v < n && parent[v] != c) {
+      const int b0 = nindex[v], e0 = nindex[v + 1];
+      if (e0 - b0 <= kChunk) {
+        beg = b0 + kAfRounds;
+        len = (e0 > beg) ? e0 - beg : 0;
+      }
+    }
+    const int incl = wave_incl_scan(len);
+    const int total = wave_bcast(incl, kWave - 1);
+    excl[lane] = incl - len;
+    rbeg[lane] = beg;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int tb = 0; tb < total; tb += kWave) {
+      const int t = tb + lane;
+      if (t < total) {
+        int lo = 0, hi = kWave;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (excl[mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        uf_unite(parent, v0 + lo, nlist[rbeg[lo] + (t - excl[lo])]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const int qsize = *bigq_count;
+  wave_expand(bigq, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
+    if (e >= load_relaxed(&nindex[en.v]) + kAfRounds && parent[en.v] != c) uf_unite(parent, en.v, nlist[e]);
+  });
+}
+
 // ------------------------------------------------------------ host side ----
 template <typename K>
 int run_cc_topo(const igx_graph* g, int* d_label, igx_run_info* info, K launch, bool need_finalize)
@@ -481,6 +616,35 @@ int style_uf(const igx_graph* g, int, void* out, const void*, igx_run_info* info
   return IGX_OK;
 }
 
+int style_afforest(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* parent = (int*)out;
+  const size_t cap = 2 * ((size_t)g->edges / kChunk) + 64;
+  void *pq = nullptr, *pc = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &pq));
+  IGX_TRY(ws_get(WS_E, sizeof(int) * 64, &pc));
+  int* counters = (int*)pc;  // [0] big chunk count, [1] sampled giant root
+  const int grid = R.persist_blocks;
+  IGX_TRY(timer_start());
+  hipLaunchKernelGGL(cc_fill, dim3(1), dim3(kBlock), 0, R.stream, counters, 0, 2LL);
+  hipLaunchKernelGGL(af_init, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nodes, parent, (Entry*)pq,
+                     counters);
+  hipLaunchKernelGGL(af_link_first, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent);
+  hipLaunchKernelGGL(uf_compress, dim3(grid), dim3(kBlock), 0, R.stream, g->nodes, parent);
+  hipLaunchKernelGGL(af_sample, dim3(1), dim3(kBlock), 0, R.stream, g->nodes, parent, counters + 1);
+  ktimer_begin("af_link_rest");
+  hipLaunchKernelGGL(af_link_rest, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent,
+                     (const Entry*)pq, (const int*)counters, (const int*)(counters + 1));
+  ktimer_end("af_link_rest");
+  hipLaunchKernelGGL(uf_compress, dim3(grid), dim3(kBlock), 0, R.stream, g->nodes, parent);
+  IGX_HIP(hipGetLastError());
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = 1;
+  info->launches = 7;
+  return IGX_OK;
+}
+
 const StyleEntry kStyles[] = {
     {"v_topo_push_thread", style_vtopo_push<kThread, kAtomicRelaxed, true>, 0},
     {"v_topo_push_wave", style_vtopo_push<kWaveG, kAtomicRelaxed, true>, 0},
@@ -493,6 +657,7 @@ const StyleEntry kStyles[] = {
     {"v_data_push_thread", style_data_push<false>, 0},
     {"e_data_push_thread", style_data_push<true>, 1},
     {"uf_hook_compress", style_uf, 0},
+    {"uf_afforest", style_afforest, 0},
 };
 
 }  // namespace

@@ -35,14 +35,24 @@ __device__ __forceinline__ void push_vertex_chunks(Entry* __restrict__ q, int* c
   const int deg = end - beg;
   const int c = pred ? chunks_of(deg) : 0;
   const int slot = wave_push_slot(counter, c == 1);
-  if (c == 1) {
-    q[slot] = Entry{v, beg, end, 0};
-  } else if (c > 1) {
-    const int base = atomicAdd(counter, c);
-    for (int k = 0; k < c; k++) {
-      const int b = beg + k * kChunk;
-      const int e = (b + kChunk < end) ? b + kChunk : end;
-      q[base + k] = Entry{v, b, e, 0};
+  if (c == 1) q[slot] = Entry{v, beg, end, 0};
+  uint64_t big = ballot(c > 1);
+  if (big != 0) {
+    // each multi-chunk vertex reserves its run of slots; then all active
+    // lanes write that run together (a hub has hundreds of chunks)
+    const int base = (c > 1) ? atomicAdd(counter, c) : 0;
+    const uint64_t act = ballot(true);
+    const int P = __popcll(act);
+    const int r = mbcnt(act);
+    while (big != 0) {
+      const int L = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const int bv = __shfl(v, L, kWave), bb = __shfl(beg, L, kWave), be = __shfl(end, L, kWave);
+      const int bbase = __shfl(base, L, kWave), bc = __shfl(c, L, kWave);
+      for (int k = r; k < bc; k += P) {
+        const int b = bb + k * kChunk;
+        q[bbase + k] = Entry{bv, b, (b + kChunk < be) ? b + kChunk : be, 0};
+      }
     }
   }
 }
@@ -51,15 +61,28 @@ __device__ __forceinline__ void push_vertex_chunks(Entry* __restrict__ q, int* c
 // `nwaves` waves (wave index `gw`).  lds_excl / lds_ent must point at this
 // wave's 64-slot LDS arrays.  visit(ent, e) is called once per edge; all lanes
 // of the wave call this function together (the loops are wave-uniform).
+// Entries per wave group: 64 when the queue is long; fewer (a power of two)
+// when it is short, so that a handful of hub chunks still spreads over all
+// waves instead of over qsize / 64 of them.
+__device__ __forceinline__ int group_entries(int qsize, int nwaves)
+{
+  if (qsize >= nwaves * kWave) return kWave;
+  const int per = (qsize + nwaves - 1) / nwaves;
+  int g = 1;
+  while (g < per) g <<= 1;
+  return g;
+}
+
 template <typename F>
 __device__ __forceinline__ void wave_expand(const Entry* __restrict__ q, int qsize, int gw, int nwaves,
                                             int* lds_excl, Entry* lds_ent, F visit)
 {
   const int lane = lane_id();
-  for (int gbase = gw * kWave; gbase < qsize; gbase += nwaves * kWave) {
+  const int G = group_entries(qsize, nwaves);
+  for (int gbase = gw * G; gbase < qsize; gbase += nwaves * G) {
     const int idx = gbase + lane;
     Entry ent = Entry{0, 0, 0, 0};
-    if (idx < qsize) ent = q[idx];
+    if (lane < G && idx < qsize) ent = q[idx];
     const int len = ent.end - ent.beg;
     const int incl = wave_incl_scan(len);
     const int total = wave_bcast(incl, kWave - 1);

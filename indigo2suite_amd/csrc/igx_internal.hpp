@@ -56,11 +56,11 @@ int ws_get(int slot, size_t bytes, void** ptr);
 // done or max_rounds were enqueued.  `launch(r)` enqueues round r.
 // Returns IGX_OK, IGX_ERR_LIMIT when max_rounds did not converge, or an error.
 template <typename F>
-int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches)
+int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches, int first_batch = 2)
 {
   Runtime& R = rt();
   long long launched = 0;
-  int batch = 2;
+  int batch = first_batch;
   while (true) {
     const long long k = (launched + batch > max_rounds) ? (max_rounds - launched) : batch;
     for (long long i = 0; i < k; i++) {

@@ -178,9 +178,9 @@ static void init_entry_styles()
   // defaults: fastest measured vertex-kind and edge-kind styles
   g_entry_style[IGX_BFS][0] = find_style(IGX_BFS, "do_hybrid");
   g_entry_style[IGX_BFS][1] = find_style(IGX_BFS, "e_topo_push_thread");
-  g_entry_style[IGX_SSSP][0] = find_style(IGX_SSSP, "v_data_push_balanced");
+  g_entry_style[IGX_SSSP][0] = find_style(IGX_SSSP, "v_do_balanced");
   g_entry_style[IGX_SSSP][1] = find_style(IGX_SSSP, "e_data_push_thread");
-  g_entry_style[IGX_CC][0] = find_style(IGX_CC, "uf_hook_compress");
+  g_entry_style[IGX_CC][0] = find_style(IGX_CC, "uf_afforest");
   g_entry_style[IGX_CC][1] = find_style(IGX_CC, "e_topo_push_thread");
   g_entry_style[IGX_MIS][0] = find_style(IGX_MIS, "v_data_pull_thread");
   g_entry_style[IGX_MIS][1] = find_style(IGX_MIS, "e_topo_thread");

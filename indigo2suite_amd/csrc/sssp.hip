@@ -308,6 +308,163 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
   finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
+// ------------------------------------------ direction-optimizing rounds ----
+// Push rounds expand the chunk queue as sssp_bal_round does.  When the next
+// frontier's edges exceed m / kAlphaS the following round pulls instead: every
+// row takes the minimum of (dist[u] + w) over its edges -- rows of at most
+// kChunk edges 64 per wave through an LDS atomicMin per row, larger rows by
+// chunks with one global atomicMin per chunk -- and every improved vertex is
+// queued for the rounds after.  A round that improves nothing ends the run.
+constexpr int kAlphaS = 8;
+
+__device__ __forceinline__ void block_add_ll2(long long* target, long long v, long long* lds)
+{
+  v = wave_sum_ll(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) t += lds[k];
+    if (t != 0) atomicAdd((unsigned long long*)target, (unsigned long long)t);
+  }
+  __syncthreads();
+}
+
+// big rows' chunks, built once per run (the pull rounds walk them)
+__global__ void __launch_bounds__(kBlock) sssp_do_bigq(const int* __restrict__ nindex, int n,
+                                                        Entry* __restrict__ bigq, int* __restrict__ bigq_count)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < n; base += stride) {
+    const int v = base + threadIdx.x;
+    int beg = 0, end = 0;
+    if (v < n) {
+      beg = nindex[v];
+      end = nindex[v + 1];
+    }
+    push_vertex_chunks(bigq, bigq_count, (end - beg) > kChunk, v, beg, end);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ nindex,
+                                                         const int* __restrict__ nlist, const int* __restrict__ ew,
+                                                         int n, long long m, int* __restrict__ dist,
+                                                         Entry* __restrict__ q0, Entry* __restrict__ q1,
+                                                         int* __restrict__ stamp, const Entry* __restrict__ bigq,
+                                                         const int* __restrict__ bigq_count, Ctl* ctl)
+{
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
+  __shared__ int s_min[kWavesPerBlock][kWave];
+  __shared__ Entry s_ent[kWavesPerBlock][kWave];
+  __shared__ long long s_red[kWavesPerBlock];
+  if (round_done(ctl)) return;
+  const int iter = ctl->iter;
+  const int mode = ctl->mode;
+  const int p = iter & 1;
+  const Entry* q_in = p ? q1 : q0;
+  Entry* q_out = p ? q0 : q1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int tag = iter + 1;
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int gw = blockIdx.x * kWavesPerBlock + w;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  long long pushed_edges = 0;
+  if (mode == 0) {
+    const int qsize = ctl->wl_count[p];
+    wave_expand(q_in, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
+      const int nd = dist[en.v] + ew[e];
+      const int u = nlist[e];
+      bool push = false;
+      int ub = 0, ue = 0;
+      if (dist[u] > nd && atomic_min_relaxed(&dist[u], nd) > nd) {
+        push = atomic_max_relaxed(&stamp[u], tag) != tag;
+        if (push) {
+          ub = nindex[u];
+          ue = nindex[u + 1];
+          pushed_edges += ue - ub;
+        }
+      }
+      push_vertex_chunks(q_out, out_count, push && ue > ub, u, ub, ue);
+    });
+  } else {
+    int* excl = s_excl[w];
+    int* rbeg = s_beg[w];
+    int* rmin = s_min[w];
+    for (int v0 = gw * kWave; v0 < n; v0 += nwaves * kWave) {
+      const int v = v0 + lane;
+      int beg = 0, len = 0;
+      if (v < n) {
+        beg = nindex[v];
+        len = nindex[v + 1] - beg;
+        if (len > kChunk) len = 0;
+      }
+      const int incl = wave_incl_scan(len);
+      const int total = wave_bcast(incl, kWave - 1);
+      excl[lane] = incl - len;
+      rbeg[lane] = beg;
+      rmin[lane] = kInf;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      for (int tb = 0; tb < total; tb += kWave) {
+        const int t = tb + lane;
+        if (t < total) {
+          int lo = 0, hi = kWave;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (excl[mid] <= t) lo = mid;
+            else hi = mid;
+          }
+          const int e = rbeg[lo] + (t - excl[lo]);
+          const int du = dist[nlist[e]];
+          if (du != kInf) atomicMin(&rmin[lo], du + ew[e]);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      bool push = false;
+      if (len > 0 && rmin[lane] < dist[v]) {
+        dist[v] = rmin[lane];
+        push = true;
+        pushed_edges += len;
+      }
+      push_vertex_chunks(q_out, out_count, push, v, beg, beg + len);
+      __builtin_amdgcn_wave_barrier();
+    }
+    const int nbig = *bigq_count;
+    wave_expand(bigq, nbig, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
+      // lanes of one chunk reduce their minimum; the first lane of each
+      // chunk in the step applies it
+      const int du = dist[nlist[e]];
+      const int cand = (du != kInf) ? du + ew[e] : kInf;
+      if (cand < dist[en.v] && atomic_min_relaxed(&dist[en.v], cand) > cand) {
+        if (atomic_max_relaxed(&stamp[en.v], tag) != tag) {
+          const int ub = nindex[en.v], ue = nindex[en.v + 1];
+          pushed_edges += ue - ub;
+          const int c = chunks_of(ue - ub);
+          const int base = atomicAdd(out_count, c);
+          for (int k = 0; k < c; k++) {
+            const int b = ub + k * kChunk;
+            q_out[base + k] = Entry{en.v, b, (b + kChunk < ue) ? b + kChunk : ue, 0};
+          }
+        }
+      }
+    });
+  }
+  block_add_ll2(&ctl->edges_next, pushed_edges, s_red);
+  finish_round(ctl, [&] {
+    const int produced = load_relaxed(&ctl->wl_count[p ^ 1]);
+    const long long ef = load_relaxed(&ctl->edges_next);
+    ctl->wl_count[p] = 0;
+    ctl->edges_next = 0;
+    ctl->iter += 1;
+    if (produced == 0) ctl->done = 1;
+    else ctl->mode = (ef > m / kAlphaS) ? 1 : 0;
+  });
+}
+
 // ------------------------------------------------------------ host side ----
 template <typename K>
 int run_sssp_rounds(const igx_graph* g, igx_run_info* info, long long max_rounds, K launch)
@@ -434,6 +591,42 @@ int style_balanced(const igx_graph* g, int src, void* out, const void*, igx_run_
   });
 }
 
+int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* dist = (int*)out;
+  const size_t cap = (size_t)g->nodes + (size_t)g->edges / kChunk + 64;
+  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr, *pb = nullptr, *pc = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &p0));
+  IGX_TRY(ws_get(WS_E, sizeof(Entry) * cap, &p1));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * (size_t)g->nodes, &p2));
+  IGX_TRY(ws_get(WS_G, sizeof(Entry) * (2 * ((size_t)g->edges / kChunk) + 64), &pb));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * 64, &pc));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
+  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, (int*)p2, 0,
+                     (long long)g->nodes);
+  hipLaunchKernelGGL(sssp_fill, dim3(1), dim3(kBlock), 0, R.stream, (int*)pc, 0, 1LL);
+  hipLaunchKernelGGL(sssp_do_bigq, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nodes,
+                     (Entry*)pb, (int*)pc);
+  hipLaunchKernelGGL(sssp_bal_seed, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, (Entry*)p0, R.d_ctl);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        ktimer_begin("sssp_do_round");
+        hipLaunchKernelGGL(sssp_do_round, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                           g->eweight, g->nodes, (long long)g->edges, dist, (Entry*)p0, (Entry*)p1, (int*)p2,
+                           (const Entry*)pb, (const int*)pc, R.d_ctl);
+        ktimer_end("sssp_do_round");
+      },
+      sssp_round_cap(g), &rounds, &launches, 8);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
 const StyleEntry kStyles[] = {
     {"v_topo_push_thread", style_vtopo_push<kThread, kAtomicRelaxed>, 0},
     {"v_topo_push_wave", style_vtopo_push<kWaveG, kAtomicRelaxed>, 0},
@@ -446,6 +639,7 @@ const StyleEntry kStyles[] = {
     {"e_data_push_thread", style_edata_push<false>, 1},
     {"e_data_push_rw_thread", style_edata_push<true>, 1},
     {"v_data_push_balanced", style_balanced, 0},
+    {"v_do_balanced", style_do, 0},
 };
 
 }  // namespace

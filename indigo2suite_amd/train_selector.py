@@ -1,0 +1,104 @@
+"""Collect style timings, train the per-algorithm selectors on the GPU, and
+evaluate them on held-out graphs.
+
+  python -m indigo2suite_amd.train_selector collect --count N --seed-base S \
+        [--log-n LO HI] [--deg LO HI] --out FILE.npz
+  python -m indigo2suite_amd.train_selector fit --train FILE.npz --out-dir DIR
+  python -m indigo2suite_amd.train_selector eval --test FILE.npz --model-dir DIR [--json OUT]
+
+`collect` runs the exhaustive autotuner (every registered style of every
+algorithm, median of three runs) on seeded graphs from four families;
+`fit` trains one MLP per algorithm with the HIP kernels; `eval` reports, per
+algorithm, the regret of the learned pick against the exhaustive best and its
+speedup over the fixed default style (the library's entry style).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import autotune, device
+from .selector import Selector
+
+ALGOS = ("bfs", "sssp", "cc", "mis", "pr", "tc")
+
+
+def cmd_collect(a):
+    t0 = time.time()
+    data = autotune.collect(a.count, ALGOS, seed_base=a.seed_base, log_n_range=tuple(a.log_n),
+                            deg_range=tuple(a.deg), repeats=a.repeats,
+                            log=lambda s: print(s, flush=True))
+    autotune.save_dataset(a.out, data)
+    print(f"wrote {a.out}: {a.count} graphs in {time.time() - t0:.0f}s", flush=True)
+
+
+def cmd_fit(a):
+    data = autotune.load_dataset(a.train)
+    os.makedirs(a.out_dir, exist_ok=True)
+    summary = {}
+    for algo in ALGOS:
+        sel = Selector(algo, data["styles"][algo], seed=a.seed)
+        t = time.time()
+        loss = sel.fit(data["features"], data["times"][algo], epochs=a.epochs, batch=a.batch, lr=a.lr)
+        sel.save(os.path.join(a.out_dir, f"selector_{algo}.npz"))
+        summary[algo] = {"final_loss": loss, "train_seconds": round(time.time() - t, 2),
+                         "graphs": int(data["features"].shape[0])}
+        print(algo, summary[algo], flush=True)
+        sel.net.free()
+    print(json.dumps(summary), flush=True)
+
+
+def cmd_eval(a):
+    data = autotune.load_dataset(a.test)
+    device.init()
+    out = {}
+    for algo in ALGOS:
+        sel = Selector.load(os.path.join(a.model_dir, f"selector_{algo}.npz"))
+        assert sel.styles == data["styles"][algo], "style registry changed since training"
+        default = device.lib().igx_get_entry_style(device.ALGOS[algo], 0)
+        ev = sel.evaluate(data["features"], data["times"][algo], default_style=default)
+        t = np.asarray(data["times"][algo])
+        best = np.argmin(t, axis=1)
+        ev["default_style"] = sel.styles[default]
+        ev["default_regret_mean"] = float(np.mean(t[:, default] / t[np.arange(len(best)), best] - 1.0))
+        ev["best_styles"] = [sel.styles[b] for b in best]
+        out[algo] = ev
+        print(algo, {k: v for k, v in ev.items() if not isinstance(v, list)}, flush=True)
+        sel.net.free()
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("--count", type=int, required=True)
+    c.add_argument("--seed-base", type=int, default=100000)
+    c.add_argument("--log-n", type=float, nargs=2, default=(12, 21))
+    c.add_argument("--deg", type=float, nargs=2, default=(4, 32))
+    c.add_argument("--repeats", type=int, default=3)
+    c.add_argument("--out", required=True)
+    f = sub.add_parser("fit")
+    f.add_argument("--train", required=True)
+    f.add_argument("--out-dir", required=True)
+    f.add_argument("--epochs", type=int, default=300)
+    f.add_argument("--batch", type=int, default=64)
+    f.add_argument("--lr", type=float, default=2e-3)
+    f.add_argument("--seed", type=int, default=1)
+    e = sub.add_parser("eval")
+    e.add_argument("--test", required=True)
+    e.add_argument("--model-dir", required=True)
+    e.add_argument("--json")
+    a = ap.parse_args(argv)
+    {"collect": cmd_collect, "fit": cmd_fit, "eval": cmd_eval}[a.cmd](a)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
