@@ -78,6 +78,8 @@ typedef struct igx_run_info {
   int launches;                    /* kernels enqueued for the rounds */
   unsigned long long count;        /* TC: triangles */
   double residual;                 /* PR: L1 change of the last iteration (FP64) */
+  unsigned long long edges_processed;    /* styles that count their work: edges touched over all rounds */
+  unsigned long long vertices_processed; /* and rows / queue entries touched */
 } igx_run_info;
 
 /* General entry.  `dev` is a device CSR (igx_graph_upload); d_out is device

@@ -35,6 +35,8 @@ class IgxRunInfo(ctypes.Structure):
         ("launches", ctypes.c_int),
         ("count", ctypes.c_ulonglong),
         ("residual", ctypes.c_double),
+        ("edges_processed", ctypes.c_ulonglong),
+        ("vertices_processed", ctypes.c_ulonglong),
     ]
 
 

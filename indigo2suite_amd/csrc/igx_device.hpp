@@ -160,6 +160,8 @@ struct Ctl {
   long long edges_next;  // BFS: sum of degrees of the next frontier
   long long edges_unvisited;
   long long aux_ll[2];
+  long long work_edges;     // edges processed by the round kernels of this run
+  long long work_vertices;  // rows / queue entries processed by them
   double res;            // PR residual of the current round (FP64)
   double res_last;       // PR residual of the last completed round
   int n;                 // vertex count (for heuristics)

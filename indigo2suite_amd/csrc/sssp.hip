@@ -371,10 +371,12 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   const int lane = lane_id();
   const int gw = blockIdx.x * kWavesPerBlock + w;
   const int nwaves = gridDim.x * kWavesPerBlock;
-  long long pushed_edges = 0;
+  long long pushed_edges = 0, work_e = 0, work_v = 0;
   if (mode == 0) {
     const int qsize = ctl->wl_count[p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) work_v = qsize;
     wave_expand(q_in, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
+      work_e += 1;
       const int nd = dist[en.v] + ew[e];
       const int u = nlist[e];
       bool push = false;
@@ -403,6 +405,10 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       }
       const int incl = wave_incl_scan(len);
       const int total = wave_bcast(incl, kWave - 1);
+      if (lane == 0) {
+        work_e += total;
+        work_v += (n - v0 < kWave) ? n - v0 : kWave;
+      }
       excl[lane] = incl - len;
       rbeg[lane] = beg;
       rmin[lane] = kInf;
@@ -435,8 +441,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     }
     const int nbig = *bigq_count;
     wave_expand(bigq, nbig, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
-      // lanes of one chunk reduce their minimum; the first lane of each
-      // chunk in the step applies it
+      work_e += 1;
       const int du = dist[nlist[e]];
       const int cand = (du != kInf) ? du + ew[e] : kInf;
       if (cand < dist[en.v] && atomic_min_relaxed(&dist[en.v], cand) > cand) {
@@ -454,6 +459,8 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     });
   }
   block_add_ll2(&ctl->edges_next, pushed_edges, s_red);
+  block_add_ll2(&ctl->work_edges, work_e, s_red);
+  block_add_ll2(&ctl->work_vertices, work_v, s_red);
   finish_round(ctl, [&] {
     const int produced = load_relaxed(&ctl->wl_count[p ^ 1]);
     const long long ef = load_relaxed(&ctl->edges_next);
@@ -624,6 +631,8 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
   IGX_TRY(timer_stop(&info->seconds));
   info->iterations = rounds;
   info->launches = launches;
+  info->edges_processed = (unsigned long long)R.h_ctl->work_edges;
+  info->vertices_processed = (unsigned long long)R.h_ctl->work_vertices;
   return rc;
 }
 
