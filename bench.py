@@ -14,10 +14,10 @@ formula, bfs_vertex_cuda.h:163, applied to the two runs of a step).
 
 roofline: the SSSP round kernel (sssp_do_round), the step's dominant kernel,
 timed with HIP events on the library stream over the timed steps.  Its
-algorithmic bytes are counted on the device: 12 bytes per edge processed
-(neighbour id, weight, neighbour state) and 12 per row or queue entry (row
-offsets, state read and write); a pull round over the whole graph is
-12m + 12n, the SSSP figure per run.
+algorithmic bytes are counted on the device: 4 bytes per edge scanned
+(neighbour id), 8 more per edge relaxed (weight, neighbour distance) and 12
+per row or queue entry visited (row offsets, distance); a pull round that
+relaxes every edge is 12m + 12n, the SSSP figure per run.
 
 cpu_baseline: the oracle's OpenMP restatement of the reference's fastest
 OpenMP styles (SSSP and CC vertex-based topology-driven pull) on a bounded
@@ -165,7 +165,7 @@ def main():
     lib.igx_profile_kernel(ROOFLINE_KERNEL.encode())
     per = {"sssp": [], "cc": []}
     iters = {"sssp": 0, "cc": 0}
-    work_e = work_v = 0
+    work_e = work_r = work_v = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         i1, i2 = step()
@@ -173,6 +173,7 @@ def main():
         per["cc"].append(i2.seconds)
         iters["sssp"], iters["cc"] = i1.iterations, i2.iterations
         work_e += i1.edges_processed
+        work_r += i1.edges_relaxed
         work_v += i1.vertices_processed
     lib.igx_sync()
     t1 = time.perf_counter()
@@ -203,19 +204,19 @@ def main():
                                            "gteps": round(m / named.seconds * 1e-9, 3),
                                            "iterations": named.iterations}
 
-    # algorithmic bytes of the SSSP round kernel, counted on the device: 12 B
-    # per edge processed (neighbour id, weight, neighbour state) and 12 B per
-    # row or queue entry (row offsets, state read and write), summed over the
-    # timed launches, including the ones that found the run converged
+    # algorithmic bytes of the SSSP round kernel, counted on the device: 4 B
+    # per edge scanned, 8 B more per edge relaxed, 12 B per row or queue entry,
+    # summed over the timed launches (4 B each for the ones that found the run
+    # converged)
     launches = max(hook_n.value, 1)
     total_s = hook_ms.value * 1e-3
-    bytes_total = 12 * work_e + 12 * work_v + 4 * launches
+    bytes_total = 4 * work_e + 8 * work_r + 12 * work_v + 4 * launches
     achieved = bytes_total / total_s * 1e-9 if total_s > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": int(bytes_total / launches), "avg_launch_us": round(total_s / launches * 1e6, 2),
-                "launches": hook_n.value, "edges_processed_per_run": int(work_e / args.steps),
-                "rows_processed_per_run": int(work_v / args.steps)}
+                "launches": hook_n.value, "edges_scanned_per_run": int(work_e / args.steps),
+                "edges_relaxed_per_run": int(work_r / args.steps), "rows_visited_per_run": int(work_v / args.steps)}
 
     cpu = None
     if not args.no_cpu:

@@ -80,6 +80,8 @@ typedef struct igx_run_info {
   double residual;                 /* PR: L1 change of the last iteration (FP64) */
   unsigned long long edges_processed;    /* styles that count their work: edges touched over all rounds */
   unsigned long long vertices_processed; /* and rows / queue entries touched */
+  unsigned long long edges_relaxed;      /* of the edges touched, those whose weight and
+                                            neighbour state were read (SSSP) */
 } igx_run_info;
 
 /* General entry.  `dev` is a device CSR (igx_graph_upload); d_out is device
@@ -98,7 +100,7 @@ int igx_run(int algo, int style, const igx_graph* dev, int src, void* d_out, con
  * igx_profile_kernel("uf_hook") every launch of that kernel is bracketed by
  * events on the library stream; igx_profile_read() waits for the stream and
  * returns the summed duration and the number of timed launches, then resets.
- * Tags: "uf_hook", "bfs_do_level", "sssp_bal_round", "pr_bal_main",
+ * Tags: "uf_hook", "af_link_rest", "bfs_do_level", "sssp_do_round", "sssp_bal_round", "pr_bal_main",
  * "tc_oriented_count", "mis_vdata_pull".  NULL or "" disables. */
 int igx_profile_kernel(const char* tag);
 int igx_profile_read(double* total_ms, int* launches);

@@ -37,6 +37,7 @@ class IgxRunInfo(ctypes.Structure):
         ("residual", ctypes.c_double),
         ("edges_processed", ctypes.c_ulonglong),
         ("vertices_processed", ctypes.c_ulonglong),
+        ("edges_relaxed", ctypes.c_ulonglong),
     ]
 
 

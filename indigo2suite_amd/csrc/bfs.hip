@@ -243,11 +243,17 @@ __global__ void __launch_bounds__(kBlock) bfs_edata_init(int src, const int* __r
   }
 }
 
+// One thread per worklist edge.  When dst improves, all of dst's edges are
+// queued for the next round at once, so the NonDup test is kept per source
+// vertex (vstamp[dst] == tag), admitting exactly the edges that per-edge
+// stamps admit; the ids are written wave-cooperatively.
 __global__ void __launch_bounds__(kBlock) bfs_edata_push(const int* __restrict__ nindex,
                                                           const int* __restrict__ nlist, const int* __restrict__ sp,
                                                           int* __restrict__ dist, int* __restrict__ wl0,
-                                                          int* __restrict__ wl1, int* __restrict__ stamp, Ctl* ctl)
+                                                          int* __restrict__ wl1, int* __restrict__ vstamp, Ctl* ctl)
 {
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
   if (round_done(ctl)) return;
   const int iter = ctl->iter;
   const int p = iter & 1;
@@ -256,38 +262,56 @@ __global__ void __launch_bounds__(kBlock) bfs_edata_push(const int* __restrict__
   int* wl_out = p ? wl0 : wl1;
   int* out_count = &ctl->wl_count[p ^ 1];
   const int tag = iter + 1;
+  const int w = threadIdx.x / kWave;
   const int stride = gridDim.x * kBlock;
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
-    const int e = wl_in[idx];
-    const int s = dist[sp[e]];
-    if (s != kInf) {
-      const int nd = s + 1;
-      const int dst = nlist[e];
-      if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) {
-        for (int j = nindex[dst]; j < nindex[dst + 1]; j++) {
-          const bool push = atomic_max_relaxed(&stamp[j], tag) != tag;
-          const int slot = wave_push_slot(out_count, push);
-          if (push) wl_out[slot] = j;
+  for (int base = blockIdx.x * kBlock; base < count; base += stride) {  // uniform per wave
+    const int idx = base + (int)threadIdx.x;
+    int rbeg = 0, rlen = 0;
+    if (idx < count) {
+      const int e = wl_in[idx];
+      const int s = dist[sp[e]];
+      if (s != kInf) {
+        const int nd = s + 1;
+        const int dst = nlist[e];
+        if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd &&
+            atomic_max_relaxed(&vstamp[dst], tag) != tag) {
+          rbeg = nindex[dst];
+          rlen = nindex[dst + 1] - rbeg;
         }
       }
     }
+    wave_push_edge_ranges(wl_out, out_count, rbeg, rlen, s_excl[w], s_beg[w]);
   }
   finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // ------------------------------------------ direction-optimizing hybrid ----
-// Per level one kernel: top-down expansion of the chunk queue, or bottom-up
-// over every vertex.  The last block picks the next direction with Beamer's
-// rule (top-down -> bottom-up when the next frontier's edges exceed the
-// unvisited edges / kAlpha; back when the frontier has < n / kBeta vertices).
+// Per level one kernel: top-down expansion of the frontier, or bottom-up over
+// every vertex.  The last block picks the next direction with Beamer's rule
+// (top-down -> bottom-up when the next frontier's edges exceed the unvisited
+// edges / kAlpha; back when the frontier has < n / kBeta vertices).
+//
+// Frontier representation.  Three bitmaps rotate: bm[level % 3] holds the
+// current frontier, bm[(level + 1) % 3] receives the next one and
+// bm[(level + 2) % 3] (the previous frontier) is cleared for the level after.
+// The bitmaps are always complete.  A top-down level whose frontier has at
+// most kQueueEdges edges also appends the discovered vertices' chunks to a
+// segmented queue (few appends, so the queue is cheaper to read than the
+// bitmap); larger top-down levels and all bottom-up levels produce the bitmap
+// only -- no queue atomics at all.  A top-down level reads the queue when the
+// previous level wrote one (ctl->in_bits == 0), else the bitmap: 64-vertex
+// tiles whose rows of at most kChunk edges one wave expands, and the
+// precomputed chunks of the longer rows, filtered by the bitmap.
 constexpr int kAlpha = 14;
 constexpr int kBeta = 24;
+constexpr long long kQueueEdges = 1 << 16;
 
 __global__ void __launch_bounds__(kBlock) bfs_do_init(int src, const int* __restrict__ nindex, int m,
                                                        Entry* __restrict__ q0, unsigned* __restrict__ bm, int words,
                                                        Ctl* ctl)
 {
-  // bitmaps 0 and 1 start empty (2 is cleared during level 0); src is in 0
+  // bitmaps 0 and 1 start empty (2 is cleared during level 0); src is in 0;
+  // src's chunks go to queue segment 0
   const int stride = gridDim.x * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < 2 * words; i += stride) {
     bm[i] = (i == (src >> 5)) ? (1u << (src & 31)) : 0u;
@@ -300,28 +324,13 @@ __global__ void __launch_bounds__(kBlock) bfs_do_init(int src, const int* __rest
     q0[k] = Entry{src, b, (b + kChunk < end) ? b + kChunk : end, 0};
   }
   if (threadIdx.x == 0) {
-    ctl->wl_count[0] = c;
-    ctl->wl_count[1] = 0;
+    ctl_groups(ctl)[0].qcount[0] = c;
     ctl->mode = 0;
     ctl->level = 0;
-    ctl->edges_next = 0;
-    ctl->aux_ll[0] = 0;
+    ctl->in_bits = 0;
+    ctl->edges_next = end - beg;  // edges of the current frontier
     ctl->edges_unvisited = (long long)m - (end - beg);
   }
-}
-
-__device__ __forceinline__ void block_add_ll(long long* target, long long v, long long* lds)
-{
-  v = wave_sum_ll(v);
-  const int w = threadIdx.x / kWave;
-  if (lane_id() == 0) lds[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long t = 0;
-    for (int k = 0; k < kWavesPerBlock; k++) t += lds[k];
-    if (t != 0) atomicAdd((unsigned long long*)target, (unsigned long long)t);
-  }
-  __syncthreads();
 }
 
 __device__ __forceinline__ bool in_bits(const unsigned* __restrict__ bits, int w)
@@ -329,19 +338,12 @@ __device__ __forceinline__ bool in_bits(const unsigned* __restrict__ bits, int w
   return (bits[w >> 5] >> (w & 31)) & 1u;
 }
 
-// One BFS level.  Three frontier bitmaps rotate: bm[level % 3] holds the
-// current frontier, bm[(level + 1) % 3] receives the next one and
-// bm[(level + 2) % 3] (the previous frontier) is cleared for the level after.
-// Top-down: expand the chunk queue, claim undiscovered vertices with a CAS,
-// set their bits, queue their chunks.  Bottom-up: every unvisited vertex looks
-// for a neighbour in the current frontier bitmap -- its first four neighbours
-// with independent loads, the rest of a long row by the whole wave with an
-// early exit -- and the wave writes its 64 bits of the next bitmap at once.
 __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ nindex,
                                                         const int* __restrict__ nlist, int n,
-                                                        int* __restrict__ dist, Entry* __restrict__ q0,
-                                                        Entry* __restrict__ q1, unsigned* __restrict__ bm, int words,
-                                                        Ctl* ctl)
+                                                        int* __restrict__ dist, SegQ q0, SegQ q1,
+                                                        const Entry* __restrict__ bigq,
+                                                        const int* __restrict__ bigq_count,
+                                                        unsigned* __restrict__ bm, int words, Ctl* ctl)
 {
   __shared__ int s_excl[kWavesPerBlock][kWave];
   __shared__ Entry s_ent[kWavesPerBlock][kWave];
@@ -349,11 +351,14 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
   if (round_done(ctl)) return;
   const int level = ctl->level;
   const int mode = ctl->mode;
+  const int in_b = ctl->in_bits;
+  const bool out_b = (mode == 1) || (ctl->edges_next > kQueueEdges);
   const int p = level & 1;
-  const Entry* q_in = p ? q1 : q0;
-  Entry* q_out = p ? q0 : q1;
-  int* out_count = &ctl->wl_count[p ^ 1];
-  unsigned* cur = bm + (size_t)(level % 3) * words;
+  const SegQ q_in = p ? q1 : q0;
+  const SegQ q_out = p ? q0 : q1;
+  Entry* q_mine = segq_mine(q_out);
+  int* out_count = segq_counter(ctl, p ^ 1, my_group());
+  const unsigned* cur = bm + (size_t)(level % 3) * words;
   unsigned* next = bm + (size_t)((level + 1) % 3) * words;
   unsigned* stale = bm + (size_t)((level + 2) % 3) * words;
   const int nd = level + 1;
@@ -367,8 +372,7 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
   }
   long long found_v = 0, found_e = 0;
   if (mode == 0) {
-    const int qsize = ctl->wl_count[p];
-    wave_expand(q_in, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry&, int e) {
+    auto discover = [&](const Entry&, int e) {
       const int u = nlist[e];
       bool disc = false;
       int ub = 0, ue = 0;
@@ -384,8 +388,56 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
           found_e += ue - ub;
         }
       }
-      push_vertex_chunks(q_out, out_count, disc, u, ub, ue);
-    });
+      if (!out_b) push_vertex_chunks(q_mine, out_count, disc && ue > ub, u, ub, ue);
+    };
+    if (!in_b) {
+      const SegPrefix pre = segq_prefix(ctl, p);
+      wave_expand_with(pre.pre[kGroups], gw, nwaves, s_excl[w], s_ent[w],
+                       [&](int i) { return segq_at(q_in, pre, i); }, discover);
+    } else {
+      // rows of at most kChunk edges, 64-vertex tiles of the bitmap
+      for (int base = gw * kWave; base < n; base += nwaves * kWave) {
+        const int u = base + lane;
+        const bool in = (u < n) && in_bits(cur, u);
+        if (ballot(in) == 0) continue;  // uniform
+        Entry ent = Entry{0, 0, 0, 0};
+        if (in) {
+          const int b = nindex[u], e = nindex[u + 1];
+          if (e - b <= kChunk) ent = Entry{u, b, e, 0};
+        }
+        const int len = ent.end - ent.beg;
+        const int incl = wave_incl_scan(len);
+        const int total = wave_bcast(incl, kWave - 1);
+        if (total == 0) continue;  // uniform
+        __builtin_amdgcn_wave_barrier();
+        s_excl[w][lane] = incl - len;
+        s_ent[w][lane] = ent;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (int tb = 0; tb < total; tb += kWave) {
+          const int t = tb + lane;
+          if (t < total) {
+            int lo = 0, hi = kWave;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_excl[w][mid] <= t) lo = mid;
+              else hi = mid;
+            }
+            const Entry& en = s_ent[w][lo];
+            discover(en, en.beg + (t - s_excl[w][lo]));
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      // longer rows: their precomputed chunks, kept when the row is in the frontier
+      wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
+                       [&](int i) {
+                         Entry en = bigq[i];
+                         if (!in_bits(cur, en.v)) en.end = en.beg;
+                         return en;
+                       },
+                       discover);
+    }
   } else {
     constexpr int kFirst = 4;
     for (int base = gw * kWave; base < n; base += nwaves * kWave) {
@@ -427,14 +479,13 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
       const uint64_t mask = ballot(found);
       if (lane == 0) next[base >> 5] = (unsigned)mask;
       if (lane == 1) next[(base >> 5) + 1] = (unsigned)(mask >> 32);
-      push_vertex_chunks(q_out, out_count, found, u, ub, ue);
     }
   }
-  block_add_ll(&ctl->aux_ll[0], found_v, s_red);
-  block_add_ll(&ctl->edges_next, found_e, s_red);
+  block_stat_add(ctl, 0, found_v, s_red);
+  block_stat_add(ctl, 1, found_e, s_red);
   finish_round(ctl, [&] {
-    const long long nf = load_relaxed(&ctl->aux_ll[0]);
-    const long long mf = load_relaxed(&ctl->edges_next);
+    const long long nf = stat_take(ctl, 0);
+    const long long mf = stat_take(ctl, 1);
     ctl->edges_unvisited -= mf;
     if (nf == 0) {
       ctl->done = 1;
@@ -443,9 +494,10 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
     } else {
       if (nf < (long long)ctl->n / kBeta) ctl->mode = 0;
     }
-    ctl->wl_count[p] = 0;
-    ctl->aux_ll[0] = 0;
-    ctl->edges_next = 0;
+    ctl->in_bits = out_b ? 1 : 0;
+    ctl->edges_next = mf;
+    GroupSlot* gs = ctl_groups(ctl);
+    for (int g = 0; g < kGroups; g++) gs[g].qcount[p] = 0;
     ctl->level += 1;
     ctl->iter += 1;
   });
@@ -570,12 +622,12 @@ int style_edata_push(const igx_graph* g, int src, void* out, const void* aux, ig
   const int* sp = (const int*)aux;
   const size_t items = (size_t)std::max(g->edges, g->nodes);
   Bufs b;
-  IGX_TRY(get_worklists(items, items, &b));
+  IGX_TRY(get_worklists(items, (size_t)g->nodes, &b));
   IGX_TRY(timer_start());
   IGX_TRY(ctl_reset(g->nodes));
   hipLaunchKernelGGL(bfs_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
-  hipLaunchKernelGGL(fill_int, dim3(blocks_for((long long)items, kBlock)), dim3(kBlock), 0, R.stream, b.stamp, 0,
-                     (long long)items);
+  hipLaunchKernelGGL(fill_int, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, b.stamp, 0,
+                     (long long)g->nodes);
   hipLaunchKernelGGL(bfs_edata_init, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, b.wl0, R.d_ctl);
   int rounds = 0, launches = 0;
   const int rc = run_rounds(
@@ -594,14 +646,19 @@ int style_do_hybrid(const igx_graph* g, int src, void* out, const void*, igx_run
 {
   Runtime& R = rt();
   int* dist = (int*)out;
-  const size_t cap = (size_t)g->nodes + (size_t)g->edges / kChunk + 64;
+  const long long cap = (long long)g->nodes + g->edges / kChunk + 64;  // chunks of all rows
   const int words = ((g->nodes + kWave - 1) / kWave) * 2;
-  void *p0 = nullptr, *p1 = nullptr, *pb = nullptr;
-  IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &p0));
-  IGX_TRY(ws_get(WS_E, sizeof(Entry) * cap, &p1));
+  void *p0 = nullptr, *p1 = nullptr, *pb = nullptr, *pq = nullptr, *pc = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * (size_t)cap * kGroups, &p0));
+  IGX_TRY(ws_get(WS_E, sizeof(Entry) * (size_t)cap * kGroups, &p1));
   IGX_TRY(ws_get(WS_F, sizeof(unsigned) * 3 * (size_t)words, &pb));
+  // chunks of the rows longer than kChunk: at most 2 * edges / kChunk
+  IGX_TRY(ws_get(WS_G, sizeof(Entry) * (2 * (size_t)(g->edges / kChunk) + 64), &pq));
+  IGX_TRY(ws_get(WS_H, sizeof(int) * 64, &pc));
+  const SegQ q0{(Entry*)p0, cap}, q1{(Entry*)p1, cap};
   IGX_TRY(timer_start());
   IGX_TRY(ctl_reset(g->nodes));
+  IGX_TRY(big_rows(g, pq, (int*)pc));
   hipLaunchKernelGGL(bfs_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
   hipLaunchKernelGGL(bfs_do_init, dim3(blocks_for(2 * (long long)words, kBlock)), dim3(kBlock), 0, R.stream, src,
                      g->nindex, g->edges, (Entry*)p0, (unsigned*)pb, words, R.d_ctl);
@@ -610,7 +667,7 @@ int style_do_hybrid(const igx_graph* g, int src, void* out, const void*, igx_run
       [&](long long) {
         ktimer_begin("bfs_do_level");
         hipLaunchKernelGGL(bfs_do_level, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
-                           g->nodes, dist, (Entry*)p0, (Entry*)p1, (unsigned*)pb, words, R.d_ctl);
+                           g->nodes, dist, q0, q1, (const Entry*)pq, (const int*)pc, (unsigned*)pb, words, R.d_ctl);
         ktimer_end("bfs_do_level");
       },
       (long long)g->nodes + 2, &rounds, &launches, 8);

@@ -160,11 +160,16 @@ __global__ void __launch_bounds__(kBlock) cc_vdata_push(const int* __restrict__ 
   finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
+// One thread per worklist edge; when dst's label drops, all of dst's edges
+// are queued at once (NonDup per source vertex, see bfs_edata_push) and
+// written wave-cooperatively.
 __global__ void __launch_bounds__(kBlock) cc_edata_push(const int* __restrict__ nindex,
                                                          const int* __restrict__ nlist, const int* __restrict__ sp,
                                                          int* __restrict__ label, int* __restrict__ wl0,
-                                                         int* __restrict__ wl1, int* __restrict__ stamp, Ctl* ctl)
+                                                         int* __restrict__ wl1, int* __restrict__ vstamp, Ctl* ctl)
 {
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
   if (round_done(ctl)) return;
   const int iter = ctl->iter;
   const int p = iter & 1;
@@ -173,18 +178,22 @@ __global__ void __launch_bounds__(kBlock) cc_edata_push(const int* __restrict__ 
   int* wl_out = p ? wl0 : wl1;
   int* out_count = &ctl->wl_count[p ^ 1];
   const int tag = iter + 1;
+  const int w = threadIdx.x / kWave;
   const int stride = gridDim.x * kBlock;
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
-    const int e = wl_in[idx];
-    const int lv = label[sp[e]];
-    const int dst = nlist[e];
-    if (label[dst] > lv && atomic_min_relaxed(&label[dst], lv) > lv) {
-      for (int j = nindex[dst]; j < nindex[dst + 1]; j++) {
-        const bool push = atomic_max_relaxed(&stamp[j], tag) != tag;
-        const int slot = wave_push_slot(out_count, push);
-        if (push) wl_out[slot] = j;
+  for (int base = blockIdx.x * kBlock; base < count; base += stride) {  // uniform per wave
+    const int idx = base + (int)threadIdx.x;
+    int rbeg = 0, rlen = 0;
+    if (idx < count) {
+      const int e = wl_in[idx];
+      const int lv = label[sp[e]];
+      const int dst = nlist[e];
+      if (label[dst] > lv && atomic_min_relaxed(&label[dst], lv) > lv &&
+          atomic_max_relaxed(&vstamp[dst], tag) != tag) {
+        rbeg = nindex[dst];
+        rlen = nindex[dst + 1] - rbeg;
       }
     }
+    wave_push_edge_ranges(wl_out, out_count, rbeg, rlen, s_excl[w], s_beg[w]);
   }
   finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
@@ -222,25 +231,17 @@ __device__ __forceinline__ void uf_unite(int* __restrict__ parent, int a, int b)
   }
 }
 
-// parent[v] = the first neighbour if it is smaller (else v); vertices with more than kChunk
-// edges also get their edge chunks queued for the balanced hook pass.
+// parent[v] = the first neighbour if it is smaller (else v).
 __global__ void __launch_bounds__(kBlock) uf_init(const int* __restrict__ nindex, const int* __restrict__ nlist,
-                                                   int n, int* __restrict__ parent, Entry* __restrict__ bigq,
-                                                   int* __restrict__ bigq_count)
+                                                   int n, int* __restrict__ parent)
 {
   const int stride = gridDim.x * kBlock;
-  for (int base = blockIdx.x * kBlock; base < n; base += stride) {
-    const int v = base + threadIdx.x;
-    int beg = 0, end = 0;
-    if (v < n) {
-      beg = nindex[v];
-      end = nindex[v + 1];
-      // any smaller neighbour is a valid first parent; with sorted adjacency
-      // the first neighbour is the smallest one
-      const int u0 = (end > beg) ? nlist[beg] : v;
-      parent[v] = (u0 < v) ? u0 : v;
-    }
-    push_vertex_chunks(bigq, bigq_count, (end - beg) > kChunk, v, beg, end);
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const int beg = nindex[v], end = nindex[v + 1];
+    // any smaller neighbour is a valid first parent; with sorted adjacency
+    // the first neighbour is the smallest one
+    const int u0 = (end > beg) ? nlist[beg] : v;
+    parent[v] = (u0 < v) ? u0 : v;
   }
 }
 
@@ -367,40 +368,42 @@ __global__ void __launch_bounds__(kBlock) uf_compress(int n, int* __restrict__ p
 constexpr int kAfRounds = 2;
 constexpr int kAfSamples = 1024;
 
-__global__ void __launch_bounds__(kBlock) af_init(const int* __restrict__ nindex, int n, int* __restrict__ parent,
-                                                   Entry* __restrict__ bigq, int* __restrict__ bigq_count)
-{
-  const int stride = gridDim.x * kBlock;
-  for (int base = blockIdx.x * kBlock; base < n; base += stride) {
-    const int v = base + threadIdx.x;
-    int beg = 0, end = 0;
-    if (v < n) {
-      beg = nindex[v];
-      end = nindex[v + 1];
-      parent[v] = v;
-    }
-    push_vertex_chunks(bigq, bigq_count, (end - beg) > kChunk, v, beg, end);
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) af_link_first(const int* __restrict__ nindex,
-                                                         const int* __restrict__ nlist, int n,
-                                                         int* __restrict__ parent)
+// parent[v] = v's first neighbour when that is smaller: linking a vertex to a
+// smaller vertex needs no atomics while every vertex is still a root, and the
+// pointers stay monotone (parent[x] <= x).  A first neighbour that is larger
+// is linked by af_link below, with the second neighbour.
+__global__ void __launch_bounds__(kBlock) af_init(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                   int n, int* __restrict__ parent)
 {
   const int stride = gridDim.x * kBlock;
   for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
     const int beg = nindex[v], end = nindex[v + 1];
+    const int u0 = (end > beg) ? nlist[beg] : v;
+    parent[v] = (u0 < v) ? u0 : v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) af_link(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                   int n, int* __restrict__ parent)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const int beg = nindex[v], end = nindex[v + 1];
+    if (end > beg && nlist[beg] > v) uf_unite(parent, v, nlist[beg]);
     #pragma unroll
-    for (int r = 0; r < kAfRounds; r++) {
+    for (int r = 1; r < kAfRounds; r++) {
       if (beg + r < end) uf_unite(parent, v, nlist[beg + r]);
     }
   }
 }
 
 // one workgroup: the most frequent root among kAfSamples hashed vertices
+// (bitonic sort in LDS, then each run's length by a binary search for its end)
 __global__ void __launch_bounds__(kBlock) af_sample(int n, const int* __restrict__ parent, int* __restrict__ out)
 {
   __shared__ int s_key[kAfSamples];
+  __shared__ int s_len[kBlock];
+  __shared__ int s_val[kBlock];
   for (int i = threadIdx.x; i < kAfSamples; i += kBlock) {
     uint32_t h = (uint32_t)i * 2654435761u + 0x9E3779B9u;
     h ^= h >> 16;
@@ -409,7 +412,6 @@ __global__ void __launch_bounds__(kBlock) af_sample(int n, const int* __restrict
     s_key[i] = parent[(int)(h % (uint32_t)n)];
   }
   __syncthreads();
-  // bitonic sort of the sampled roots
   for (int k = 2; k <= kAfSamples; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
       for (int i = threadIdx.x; i < kAfSamples; i += kBlock) {
@@ -426,17 +428,32 @@ __global__ void __launch_bounds__(kBlock) af_sample(int n, const int* __restrict
       __syncthreads();
     }
   }
-  if (threadIdx.x == 0) {
-    int best = s_key[0], best_run = 0, run = 0;
-    for (int i = 0; i < kAfSamples; i++) {
-      run = (i > 0 && s_key[i] == s_key[i - 1]) ? run + 1 : 1;
-      if (run > best_run) {
-        best_run = run;
-        best = s_key[i];
-      }
+  int best_len = 0, best_val = s_key[0];
+  for (int i = threadIdx.x; i < kAfSamples; i += kBlock) {
+    const int x = s_key[i];
+    if (i > 0 && s_key[i - 1] == x) continue;  // not the start of a run
+    int lo = i, hi = kAfSamples;                // first index with key > x
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_key[mid] <= x) lo = mid + 1;
+      else hi = mid;
     }
-    *out = best;
+    if (lo - i > best_len) {
+      best_len = lo - i;
+      best_val = x;
+    }
   }
+  s_len[threadIdx.x] = best_len;
+  s_val[threadIdx.x] = best_val;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off && s_len[threadIdx.x + off] > s_len[threadIdx.x]) {
+      s_len[threadIdx.x] = s_len[threadIdx.x + off];
+      s_val[threadIdx.x] = s_val[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s_val[0];
 }
 
 __global__ void __launch_bounds__(kBlock) af_link_rest(const int* __restrict__ nindex, const int* __restrict__ nlist,
@@ -467,6 +484,8 @@ __global__ void __launch_bounds__(kBlock) af_link_rest(const int* __restrict__ n
     }
     const int incl = wave_incl_scan(len);
     const int total = wave_bcast(incl, kWave - 1);
+    if (total == 0) continue;  // uniform
+    __builtin_amdgcn_wave_barrier();
     excl[lane] = incl - len;
     rbeg[lane] = beg;
     __builtin_amdgcn_wave_barrier();
@@ -485,10 +504,21 @@ __global__ void __launch_bounds__(kBlock) af_link_rest(const int* __restrict__ n
     }
     __builtin_amdgcn_wave_barrier();
   }
-  const int qsize = *bigq_count;
-  wave_expand(bigq, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
-    if (e >= load_relaxed(&nindex[en.v]) + kAfRounds && parent[en.v] != c) uf_unite(parent, en.v, nlist[e]);
-  });
+  // longer rows by chunks; a chunk whose row is in the sampled component (at
+  // the time it is fetched) is skipped whole, the first kAfRounds edges of a
+  // row were linked before
+  wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
+                   [&](int i) {
+                     Entry en = bigq[i];
+                     if (parent[en.v] == c) {
+                       en.end = en.beg;
+                     } else {
+                       const int first = nindex[en.v] + kAfRounds;
+                       if (en.beg < first) en.beg = (first < en.end) ? first : en.end;
+                     }
+                     return en;
+                   },
+                   [&](const Entry& en, int e) { uf_unite(parent, en.v, nlist[e]); });
 }
 
 // ------------------------------------------------------------ host side ----
@@ -557,7 +587,7 @@ int style_data_push(const igx_graph* g, int, void* out, const void* aux, igx_run
   const int* sp = (const int*)aux;
   const size_t items = (size_t)std::max(g->edges, g->nodes);
   const int seed_count = EDGE ? g->edges : g->nodes;
-  const size_t stamp_items = EDGE ? items : (size_t)g->nodes;
+  const size_t stamp_items = (size_t)g->nodes;  // per-vertex NonDup stamps in both kinds
   void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
   IGX_TRY(ws_get(WS_A, sizeof(int) * items, &p0));
   IGX_TRY(ws_get(WS_B, sizeof(int) * items, &p1));
@@ -602,8 +632,9 @@ int style_uf(const igx_graph* g, int, void* out, const void*, igx_run_info* info
   const int grid = R.persist_blocks;
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(cc_fill, dim3(1), dim3(kBlock), 0, R.stream, (int*)pc, 0, 1LL);
-  hipLaunchKernelGGL(uf_init, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent,
-                     (Entry*)pq, (int*)pc);
+  hipLaunchKernelGGL(uf_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                     g->nodes, parent);
+  IGX_TRY(big_rows(g, pq, (int*)pc));
   ktimer_begin("uf_hook");
   hipLaunchKernelGGL(uf_hook, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent,
                      (const Entry*)pq, (const int*)pc);
@@ -612,7 +643,7 @@ int style_uf(const igx_graph* g, int, void* out, const void*, igx_run_info* info
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
   info->iterations = 1;
-  info->launches = 4;
+  info->launches = 5;
   return IGX_OK;
 }
 
@@ -620,6 +651,7 @@ int style_afforest(const igx_graph* g, int, void* out, const void*, igx_run_info
 {
   Runtime& R = rt();
   int* parent = (int*)out;
+  // chunks of the rows longer than kChunk: at most 2 * edges / kChunk
   const size_t cap = 2 * ((size_t)g->edges / kChunk) + 64;
   void *pq = nullptr, *pc = nullptr;
   IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &pq));
@@ -628,9 +660,10 @@ int style_afforest(const igx_graph* g, int, void* out, const void*, igx_run_info
   const int grid = R.persist_blocks;
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(cc_fill, dim3(1), dim3(kBlock), 0, R.stream, counters, 0, 2LL);
-  hipLaunchKernelGGL(af_init, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nodes, parent, (Entry*)pq,
-                     counters);
-  hipLaunchKernelGGL(af_link_first, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent);
+  hipLaunchKernelGGL(af_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                     g->nodes, parent);
+  IGX_TRY(big_rows(g, pq, counters));
+  hipLaunchKernelGGL(af_link, dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes, parent);
   hipLaunchKernelGGL(uf_compress, dim3(grid), dim3(kBlock), 0, R.stream, g->nodes, parent);
   hipLaunchKernelGGL(af_sample, dim3(1), dim3(kBlock), 0, R.stream, g->nodes, parent, counters + 1);
   ktimer_begin("af_link_rest");
@@ -641,7 +674,7 @@ int style_afforest(const igx_graph* g, int, void* out, const void*, igx_run_info
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
   info->iterations = 1;
-  info->launches = 7;
+  info->launches = 8;
   return IGX_OK;
 }
 

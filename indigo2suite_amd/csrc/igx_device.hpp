@@ -153,11 +153,11 @@ struct Ctl {
   int done;              // 1 once converged
   int iter;              // rounds completed
   int flag;              // "goagain" of the current round
-  unsigned arrived;      // blocks finished in the current round
+  unsigned arrived;      // groups finished in the current round
   int wl_count[2];       // worklist sizes, indexed by round parity
   int mode;              // algorithm-specific (BFS: 0 top-down, 1 bottom-up)
   int level;             // BFS level being expanded
-  long long edges_next;  // BFS: sum of degrees of the next frontier
+  long long edges_next;  // BFS: edges of the current frontier; SSSP: of the next one
   long long edges_unvisited;
   long long aux_ll[2];
   long long work_edges;     // edges processed by the round kernels of this run
@@ -165,26 +165,89 @@ struct Ctl {
   double res;            // PR residual of the current round (FP64)
   double res_last;       // PR residual of the last completed round
   int n;                 // vertex count (for heuristics)
-  int pad;
+  int in_bits;           // BFS: the current frontier is held by its bitmap only
 };
 
+// Same-address atomics from thousands of workgroups serialise (one every
+// ~15 ns on MI355X), so a round's arrivals, statistics and queue counters are
+// spread over kGroups slots, each on its own 4 KiB page: workgroup b uses slot
+// b % kGroups, which is also the XCD that the hardware deals it to.  The slots
+// follow the control block in the same allocation.
+constexpr int kGroups = 8;
+constexpr int kCtlBytes = 4096;
+constexpr int kStats = 4;
+
+struct alignas(4096) GroupSlot {
+  unsigned arrived;            // workgroups of this group finished in the round
+  unsigned pad0[31];
+  long long stat[kStats];      // per-round partial sums (reset by the finaliser)
+  long long pad1[12];
+  int qcount[2];               // queue segment sizes, indexed by round parity
+  int pad2[30];
+};
+static_assert(sizeof(Ctl) <= kCtlBytes, "control block exceeds its page");
+constexpr size_t kCtlAllocBytes = kCtlBytes + kGroups * sizeof(GroupSlot);
+
+__device__ __forceinline__ GroupSlot* ctl_groups(Ctl* ctl)
+{
+  return reinterpret_cast<GroupSlot*>(reinterpret_cast<char*>(ctl) + kCtlBytes);
+}
+__device__ __forceinline__ int my_group() { return (int)(blockIdx.x % kGroups); }
+
 // Block-level epilogue of one round.  All threads of the block must call it.
-// `finalize` runs on exactly one thread of the last block to arrive, after a
-// device-scope fence, and must reset per-round fields it uses.
+// Arrival is two-level: a block counts in its group's slot; the last block of
+// a group counts in ctl->arrived; `finalize` runs on exactly one thread of the
+// last block overall, after device-scope fences, and must reset per-round
+// fields it uses (group stats: stat_take).
 template <typename F>
 __device__ __forceinline__ void finish_round(Ctl* ctl, F finalize)
 {
   __syncthreads();
   if (threadIdx.x == 0) {
+    const unsigned nb = gridDim.x;
+    const unsigned grp = blockIdx.x % kGroups;
+    const unsigned ngroups = nb < (unsigned)kGroups ? nb : (unsigned)kGroups;
+    const unsigned gsize = (nb - 1 - grp) / kGroups + 1;
+    GroupSlot* gs = ctl_groups(ctl) + grp;
     __threadfence();
-    const unsigned prev = atomicAdd(&ctl->arrived, 1u);
-    if (prev == gridDim.x - 1) {
+    if (atomicAdd(&gs->arrived, 1u) == gsize - 1) {
+      gs->arrived = 0;  // every block of the group has arrived: reusable next round
       __threadfence();
-      finalize();
-      ctl->arrived = 0;
-      __threadfence();
+      if (atomicAdd(&ctl->arrived, 1u) == ngroups - 1) {
+        __threadfence();
+        finalize();
+        ctl->arrived = 0;
+        __threadfence();
+      }
     }
   }
+}
+
+// Add a block's value to statistic k of its group (one atomic per block).
+__device__ __forceinline__ void block_stat_add(Ctl* ctl, int k, long long v, long long* lds4)
+{
+  v = wave_sum_ll(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) lds4[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += lds4[i];  // lds4: one slot per wave
+    if (t != 0) atomicAdd((unsigned long long*)&ctl_groups(ctl)[my_group()].stat[k], (unsigned long long)t);
+  }
+  __syncthreads();
+}
+
+// Finaliser side: the round's total of statistic k, clearing the partials.
+__device__ __forceinline__ long long stat_take(Ctl* ctl, int k)
+{
+  GroupSlot* gs = ctl_groups(ctl);
+  long long t = 0;
+  for (int g = 0; g < kGroups; g++) {
+    t += load_relaxed(&gs[g].stat[k]);
+    gs[g].stat[k] = 0;
+  }
+  return t;
 }
 
 // the common topology-driven rule: stop after a round without updates

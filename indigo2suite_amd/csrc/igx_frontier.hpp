@@ -27,8 +27,10 @@ struct Entry {
 __device__ __forceinline__ int chunks_of(int deg) { return (deg + kChunk - 1) / kChunk; }
 
 // Push the chunks of vertex v (edge range [beg, end)) into q.  Lanes with
-// exactly one chunk share one wave-aggregated atomic; larger vertices reserve
-// their own slots.  Safe in divergent code.
+// exactly one chunk share one wave-aggregated atomic, lanes with several
+// share a second one (their runs are laid out in lane order), then all active
+// lanes write the multi-chunk runs together (a hub has hundreds of chunks).
+// Safe in divergent code.
 __device__ __forceinline__ void push_vertex_chunks(Entry* __restrict__ q, int* counter, bool pred, int v, int beg,
                                                    int end)
 {
@@ -38,9 +40,18 @@ __device__ __forceinline__ void push_vertex_chunks(Entry* __restrict__ q, int* c
   if (c == 1) q[slot] = Entry{v, beg, end, 0};
   uint64_t big = ballot(c > 1);
   if (big != 0) {
-    // each multi-chunk vertex reserves its run of slots; then all active
-    // lanes write that run together (a hub has hundreds of chunks)
-    const int base = (c > 1) ? atomicAdd(counter, c) : 0;
+    // lane-ordered offsets of the multi-chunk runs (serial over those lanes)
+    const int lane = lane_id();
+    int total = 0, off = 0;
+    for (uint64_t b = big; b != 0; b &= b - 1) {
+      const int L = __ffsll((unsigned long long)b) - 1;
+      if (lane == L) off = total;
+      total += __shfl(c, L, kWave);
+    }
+    const int leader = __ffsll((unsigned long long)big) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader, kWave) + off;
     const uint64_t act = ballot(true);
     const int P = __popcll(act);
     const int r = mbcnt(act);
@@ -55,6 +66,80 @@ __device__ __forceinline__ void push_vertex_chunks(Entry* __restrict__ q, int* c
       }
     }
   }
+}
+
+// ----------------------------------------------------- segmented queues ----
+// A chunk queue split into kGroups segments of `cap` entries each.  Workgroup
+// b appends to segment b % kGroups, whose size lives in that group's control
+// slot, so pushes from the whole grid spread over kGroups counters; readers
+// walk the concatenation of the segments.  Each segment can hold a full
+// round's output on its own.
+struct SegQ {
+  Entry* base;
+  long long cap;
+};
+
+__device__ __forceinline__ Entry* segq_mine(const SegQ& q) { return q.base + (long long)my_group() * q.cap; }
+__device__ __forceinline__ int* segq_counter(Ctl* ctl, int parity, int seg)
+{
+  return &ctl_groups(ctl)[seg].qcount[parity];
+}
+
+// prefix offsets of the kGroups segment sizes (kept in registers by the reader)
+struct SegPrefix {
+  int pre[kGroups + 1];
+};
+__device__ __forceinline__ SegPrefix segq_prefix(Ctl* ctl, int parity)
+{
+  SegPrefix sp;
+  sp.pre[0] = 0;
+  #pragma unroll
+  for (int g = 0; g < kGroups; g++) sp.pre[g + 1] = sp.pre[g] + load_relaxed(segq_counter(ctl, parity, g));
+  return sp;
+}
+__device__ __forceinline__ Entry segq_at(const SegQ& q, const SegPrefix& sp, int idx)
+{
+  int s = 0;
+  #pragma unroll
+  for (int g = 1; g < kGroups; g++) s += (idx >= sp.pre[g]) ? 1 : 0;
+  return q.base[(long long)s * q.cap + (idx - sp.pre[s])];
+}
+
+// Append the edge ids [beg, beg + len) of every lane to an edge worklist:
+// one atomic per wave reserves the wave's total, then the 64 lanes write the
+// concatenated ranges together (each lane finds its range by binary search
+// over the exclusive prefixes in LDS), so a hub's ids go out 64 at a time in
+// contiguous stores.  All lanes of the wave call this together; lanes with
+// nothing to push pass len = 0.  lds_excl / lds_beg: this wave's 64-slot LDS
+// rows.
+__device__ __forceinline__ void wave_push_edge_ranges(int* __restrict__ out, int* counter, int beg, int len,
+                                                      int* lds_excl, int* lds_beg)
+{
+  const int lane = lane_id();
+  const int incl = wave_incl_scan(len);
+  const int total = wave_bcast(incl, kWave - 1);
+  if (total == 0) return;
+  int base = 0;
+  if (lane == kWave - 1) base = atomicAdd(counter, total);
+  base = wave_bcast(base, kWave - 1);
+  __builtin_amdgcn_wave_barrier();
+  lds_excl[lane] = incl - len;
+  lds_beg[lane] = beg;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  for (int tb = 0; tb < total; tb += kWave) {
+    const int t = tb + lane;
+    if (t < total) {
+      int lo = 0, hi = kWave;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lds_excl[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      out[base + t] = lds_beg[lo] + (t - lds_excl[lo]);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Walk all edges of the queue entries [0, qsize) with the calling wave among
@@ -73,19 +158,23 @@ __device__ __forceinline__ int group_entries(int qsize, int nwaves)
   return g;
 }
 
-template <typename F>
-__device__ __forceinline__ void wave_expand(const Entry* __restrict__ q, int qsize, int gw, int nwaves,
-                                            int* lds_excl, Entry* lds_ent, F visit)
+// Expansion over any indexable source of entries: fetch(idx) returns entry
+// idx of [0, qsize); entries with end == beg are skipped at no cost.
+template <typename Fetch, typename F>
+__device__ __forceinline__ void wave_expand_with(int qsize, int gw, int nwaves, int* lds_excl, Entry* lds_ent,
+                                                 Fetch fetch, F visit)
 {
   const int lane = lane_id();
   const int G = group_entries(qsize, nwaves);
   for (int gbase = gw * G; gbase < qsize; gbase += nwaves * G) {
     const int idx = gbase + lane;
     Entry ent = Entry{0, 0, 0, 0};
-    if (lane < G && idx < qsize) ent = q[idx];
+    if (lane < G && idx < qsize) ent = fetch(idx);
     const int len = ent.end - ent.beg;
     const int incl = wave_incl_scan(len);
     const int total = wave_bcast(incl, kWave - 1);
+    if (total == 0) continue;  // uniform
+    __builtin_amdgcn_wave_barrier();
     lds_excl[lane] = incl - len;
     lds_ent[lane] = ent;
     __builtin_amdgcn_wave_barrier();
@@ -106,6 +195,13 @@ __device__ __forceinline__ void wave_expand(const Entry* __restrict__ q, int qsi
     }
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+template <typename F>
+__device__ __forceinline__ void wave_expand(const Entry* __restrict__ q, int qsize, int gw, int nwaves,
+                                            int* lds_excl, Entry* lds_ent, F visit)
+{
+  wave_expand_with(qsize, gw, nwaves, lds_excl, lds_ent, [&](int i) { return q[i]; }, visit);
 }
 
 }  // namespace igx

@@ -48,7 +48,7 @@ int ensure_init();
 // Cached device buffers, one per slot, grown on demand and kept until
 // igx_release_workspace() -- allocation stays outside the timed region.
 enum WsSlot {
-  WS_A = 0, WS_B, WS_C, WS_D, WS_E, WS_F, WS_G, WS_H, WS_SP, WS_SCRATCH, WS_COUNT
+  WS_A = 0, WS_B, WS_C, WS_D, WS_E, WS_F, WS_G, WS_H, WS_SP, WS_SCRATCH, WS_BLOCKS, WS_COUNT
 };
 int ws_get(int slot, size_t bytes, void** ptr);
 
@@ -122,6 +122,13 @@ PrParams& pr_params();
 
 // device sp[] of the graph, built on the device and cached per graph pointer
 int device_sp(const igx_graph* g, const int** d_sp);
+
+// Enqueue the construction of the chunk list of every row longer than the
+// frontier chunk (igx_frontier.hpp): bigq receives the chunks, *d_count their
+// number.  Two kernels, no atomics: per-block counts, then each block writes
+// at the sum of the counts before it.  bigq needs 2 * edges / chunk + 64
+// entries.
+int big_rows(const igx_graph* g, void* bigq, int* d_count);
 
 // events bracketing the timed region
 int timer_start();

@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "igx_internal.hpp"
+#include "igx_frontier.hpp"
 
 namespace igx {
 
@@ -36,6 +37,13 @@ __global__ void ctl_reset_kernel(Ctl* ctl, int n)
     Ctl c = {};
     c.n = n;
     *ctl = c;
+  }
+  if ((int)threadIdx.x < kGroups) {
+    GroupSlot* gs = ctl_groups(ctl) + threadIdx.x;
+    gs->arrived = 0;
+    for (int k = 0; k < kStats; k++) gs->stat[k] = 0;
+    gs->qcount[0] = 0;
+    gs->qcount[1] = 0;
   }
 }
 
@@ -111,6 +119,108 @@ int device_sp(const igx_graph* g, const int** d_sp)
   g_sp_key = g->nlist;
   g_sp_edges = g->edges;
   *d_sp = (const int*)p;
+  return IGX_OK;
+}
+
+// ---------------------------------------------------- long-row chunk list
+// Block b owns the contiguous rows [b * span, (b + 1) * span), span a
+// multiple of the block size, and scans them twice: big_rows_count stores the
+// number of chunks of its long rows, big_rows_write sums the counts of the
+// blocks before it and writes its chunks there, in row order.
+__device__ __forceinline__ void big_rows_span(int n, int* v0, int* v1)
+{
+  const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int span = (per + kBlock - 1) / kBlock * kBlock;
+  const long long lo = (long long)blockIdx.x * span;
+  *v0 = (lo < n) ? (int)lo : n;
+  *v1 = (n - *v0 < span) ? n : *v0 + span;
+}
+
+__global__ void __launch_bounds__(kBlock) big_rows_count(const int* __restrict__ nindex, int n,
+                                                          int* __restrict__ block_counts)
+{
+  __shared__ int s_wave[kWavesPerBlock];
+  int v0, v1;
+  big_rows_span(n, &v0, &v1);
+  int mine = 0;
+  for (int v = v0 + (int)threadIdx.x; v < v1; v += kBlock) {
+    const int d = nindex[v + 1] - nindex[v];
+    if (d > kChunk) mine += chunks_of(d);
+  }
+  mine = wave_sum(mine);
+  if (lane_id() == 0) s_wave[threadIdx.x / kWave] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) t += s_wave[k];
+    block_counts[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) big_rows_write(const int* __restrict__ nindex, int n,
+                                                          const int* __restrict__ block_counts,
+                                                          Entry* __restrict__ bigq, int* __restrict__ count)
+{
+  __shared__ int s_wave[kWavesPerBlock];
+  __shared__ int s_wave2[kWavesPerBlock];
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  // offset of this block (and, in block 0, the total)
+  int before = 0, all = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+    const int c = block_counts[b];
+    all += c;
+    if (b < (int)blockIdx.x) before += c;
+  }
+  before = wave_sum(before);
+  all = wave_sum(all);
+  if (lane == 0) {
+    s_wave[w] = before;
+    s_wave2[w] = all;
+  }
+  __syncthreads();
+  int run = 0, tot = 0;
+  for (int k = 0; k < kWavesPerBlock; k++) {
+    run += s_wave[k];
+    tot += s_wave2[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = tot;
+  int v0, v1;
+  big_rows_span(n, &v0, &v1);
+  for (int b0 = v0; b0 < v1; b0 += kBlock) {  // uniform per block
+    const int v = b0 + (int)threadIdx.x;
+    int c = 0, beg = 0, end = 0;
+    if (v < v1) {
+      beg = nindex[v];
+      end = nindex[v + 1];
+      if (end - beg > kChunk) c = chunks_of(end - beg);
+    }
+    const int incl = wave_incl_scan(c);
+    __syncthreads();
+    if (lane == kWave - 1) s_wave[w] = incl;
+    __syncthreads();
+    int woff = 0, btot = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      if (k < w) woff += s_wave[k];
+      btot += s_wave[k];
+    }
+    const int pos = run + woff + incl - c;
+    for (int k = 0; k < c; k++) {
+      const int b = beg + k * kChunk;
+      bigq[pos + k] = Entry{v, b, (b + kChunk < end) ? b + kChunk : end, 0};
+    }
+    run += btot;
+  }
+}
+
+int big_rows(const igx_graph* g, void* bigq, int* d_count)
+{
+  const int grid = g_rt.persist_blocks;
+  void* pb = nullptr;
+  IGX_TRY(ws_get(WS_BLOCKS, sizeof(int) * (size_t)grid, &pb));
+  hipLaunchKernelGGL(big_rows_count, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes, (int*)pb);
+  hipLaunchKernelGGL(big_rows_write, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes,
+                     (const int*)pb, (Entry*)bigq, d_count);
   return IGX_OK;
 }
 
@@ -217,7 +327,8 @@ int igx_init(int device)
   IGX_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
   IGX_HIP(hipEventCreate(&g_rt.ev_start));
   IGX_HIP(hipEventCreate(&g_rt.ev_stop));
-  IGX_HIP(hipMalloc((void**)&g_rt.d_ctl, sizeof(Ctl)));
+  IGX_HIP(hipMalloc((void**)&g_rt.d_ctl, kCtlAllocBytes));
+  IGX_HIP(hipMemset(g_rt.d_ctl, 0, kCtlAllocBytes));
   IGX_HIP(hipHostMalloc((void**)&g_rt.h_ctl, sizeof(Ctl), hipHostMallocDefault));
   memset(g_rt.h_ctl, 0, sizeof(Ctl));
   g_rt.ready = true;

@@ -208,13 +208,24 @@ __global__ void __launch_bounds__(kBlock) sssp_edata_seed(int src, const int* __
   }
 }
 
+// One thread per worklist edge.  An edge of dst is queued for the next round
+// when dst improves; every edge of dst is queued together, so the NonDup test
+// is kept per source vertex (vstamp[dst] == tag: dst's edges are queued this
+// round), which admits exactly the edges the per-edge stamps admit.  The
+// queueing itself is wave-cooperative (wave_push_edge_ranges): a hub's edge
+// ids are written 64 per step instead of one by one by the improving thread.
+// RW: the racy load/store may lose an update to a concurrent store, so the
+// edge itself is queued again (per-edge stamp estamp).
 template <bool RW>
 __global__ void __launch_bounds__(kBlock) sssp_edata_push(const int* __restrict__ nindex,
                                                            const int* __restrict__ nlist,
                                                            const int* __restrict__ ew, const int* __restrict__ sp,
                                                            int* __restrict__ dist, int* __restrict__ wl0,
-                                                           int* __restrict__ wl1, int* __restrict__ stamp, Ctl* ctl)
+                                                           int* __restrict__ wl1, int* __restrict__ vstamp,
+                                                           int* __restrict__ estamp, Ctl* ctl)
 {
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
   if (round_done(ctl)) return;
   const int iter = ctl->iter;
   const int p = iter & 1;
@@ -223,34 +234,37 @@ __global__ void __launch_bounds__(kBlock) sssp_edata_push(const int* __restrict_
   int* wl_out = p ? wl0 : wl1;
   int* out_count = &ctl->wl_count[p ^ 1];
   const int tag = iter + 1;
+  const int w = threadIdx.x / kWave;
   const int stride = gridDim.x * kBlock;
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
-    const int e = wl_in[idx];
-    const int s = load_relaxed(&dist[sp[e]]);
-    if (s != kInf) {
-      const int nd = s + ew[e];
-      const int dst = nlist[e];
-      bool improved;
-      if (RW) {
-        improved = load_relaxed(&dist[dst]) > nd;
-        if (improved) {
-          store_relaxed(&dist[dst], nd);
-          // a concurrent store may overwrite this one: re-queue e itself
-          const bool push = atomic_max_relaxed(&stamp[e], tag) != tag;
-          const int slot = wave_push_slot(out_count, push);
-          if (push) wl_out[slot] = e;
+  for (int base = blockIdx.x * kBlock; base < count; base += stride) {  // uniform per wave
+    const int idx = base + (int)threadIdx.x;
+    int rbeg = 0, rlen = 0;
+    if (idx < count) {
+      const int e = wl_in[idx];
+      const int s = load_relaxed(&dist[sp[e]]);
+      if (s != kInf) {
+        const int nd = s + ew[e];
+        const int dst = nlist[e];
+        bool improved;
+        if (RW) {
+          improved = load_relaxed(&dist[dst]) > nd;
+          if (improved) {
+            store_relaxed(&dist[dst], nd);
+            if (atomic_max_relaxed(&estamp[e], tag) != tag) {
+              const int slot = atomicAdd(out_count, 1);
+              wl_out[slot] = e;
+            }
+          }
+        } else {
+          improved = dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd;
         }
-      } else {
-        improved = dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd;
-      }
-      if (improved) {
-        for (int j = nindex[dst]; j < nindex[dst + 1]; j++) {
-          const bool push = atomic_max_relaxed(&stamp[j], tag) != tag;
-          const int slot = wave_push_slot(out_count, push);
-          if (push) wl_out[slot] = j;
+        if (improved && atomic_max_relaxed(&vstamp[dst], tag) != tag) {
+          rbeg = nindex[dst];
+          rlen = nindex[dst + 1] - rbeg;
         }
       }
     }
+    wave_push_edge_ranges(wl_out, out_count, rbeg, rlen, s_excl[w], s_beg[w]);
   }
   finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
@@ -309,50 +323,67 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
 }
 
 // ------------------------------------------ direction-optimizing rounds ----
-// Push rounds expand the chunk queue as sssp_bal_round does.  When the next
-// frontier's edges exceed m / kAlphaS the following round pulls instead: every
-// row takes the minimum of (dist[u] + w) over its edges -- rows of at most
-// kChunk edges 64 per wave through an LDS atomicMin per row, larger rows by
-// chunks with one global atomicMin per chunk -- and every improved vertex is
-// queued for the rounds after.  A round that improves nothing ends the run.
+// The frontier of a round is the set of vertices whose distance dropped in
+// the round before; it is always held by a bitmap (three rotate:
+// bm[iter % 3] current, bm[(iter + 1) % 3] next, bm[(iter + 2) % 3] cleared
+// for the round after), and a push round whose frontier has at most
+// kQueueEdgesS edges also lists the next frontier's chunks in a segmented
+// queue.  The first setter of a vertex's bit (atomicOr's old value) is the
+// one that counts and queues it, so no stamps are needed.
+//
+// Push round: relax the frontier's edges, read from the queue when the round
+// before wrote one, else from the bitmap (64-vertex tiles of rows with at most
+// kChunk edges, one wave per tile, plus the precomputed chunks of the longer
+// rows that are in the frontier).
+// Pull round: every row takes the minimum of dist[u] + w over the neighbours u
+// in the frontier (the only ones that can lower it; the others were taken
+// into account when they last changed) -- rows of at most kChunk edges 64 per
+// wave through an LDS atomicMin per row, the longer ones by their chunks with
+// a global atomicMin.
+// The next round pulls when the next frontier's edges exceed m / kAlphaS.  A
+// round that lowers nothing ends the run.
+//
+// Device work counters (algorithmic bytes of the round kernel): edges scanned
+// (4 B id each), edges relaxed (4 B weight + 4 B neighbour distance more),
+// rows visited (8 B offsets + 4 B distance).
 constexpr int kAlphaS = 8;
+constexpr long long kQueueEdgesS = 1 << 16;
 
-__device__ __forceinline__ void block_add_ll2(long long* target, long long v, long long* lds)
-{
-  v = wave_sum_ll(v);
-  const int w = threadIdx.x / kWave;
-  if (lane_id() == 0) lds[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long t = 0;
-    for (int k = 0; k < kWavesPerBlock; k++) t += lds[k];
-    if (t != 0) atomicAdd((unsigned long long*)target, (unsigned long long)t);
-  }
-  __syncthreads();
-}
+enum { kStatFrontierEdges = 0, kStatScanned = 1, kStatRelaxed = 2, kStatRows = 3 };
 
-// big rows' chunks, built once per run (the pull rounds walk them)
-__global__ void __launch_bounds__(kBlock) sssp_do_bigq(const int* __restrict__ nindex, int n,
-                                                        Entry* __restrict__ bigq, int* __restrict__ bigq_count)
+__global__ void __launch_bounds__(kBlock) sssp_do_init(int src, const int* __restrict__ nindex, Entry* __restrict__ q0,
+                                                        unsigned* __restrict__ bm, int words, Ctl* ctl)
 {
   const int stride = gridDim.x * kBlock;
-  for (int base = blockIdx.x * kBlock; base < n; base += stride) {
-    const int v = base + threadIdx.x;
-    int beg = 0, end = 0;
-    if (v < n) {
-      beg = nindex[v];
-      end = nindex[v + 1];
-    }
-    push_vertex_chunks(bigq, bigq_count, (end - beg) > kChunk, v, beg, end);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < 2 * words; i += stride) {
+    bm[i] = (i == (src >> 5)) ? (1u << (src & 31)) : 0u;
   }
+  if (blockIdx.x != 0) return;
+  const int beg = nindex[src], end = nindex[src + 1];
+  const int c = chunks_of(end - beg);
+  for (int k = threadIdx.x; k < c; k += kBlock) {
+    const int b = beg + k * kChunk;
+    q0[k] = Entry{src, b, (b + kChunk < end) ? b + kChunk : end, 0};
+  }
+  if (threadIdx.x == 0) {
+    ctl_groups(ctl)[0].qcount[0] = c;
+    ctl->mode = 0;
+    ctl->in_bits = 0;
+    ctl->edges_next = end - beg;  // edges of the current frontier
+  }
+}
+
+__device__ __forceinline__ bool bit_of(const unsigned* __restrict__ bits, int v)
+{
+  return (bits[v >> 5] >> (v & 31)) & 1u;
 }
 
 __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ nindex,
                                                          const int* __restrict__ nlist, const int* __restrict__ ew,
-                                                         int n, long long m, int* __restrict__ dist,
-                                                         Entry* __restrict__ q0, Entry* __restrict__ q1,
-                                                         int* __restrict__ stamp, const Entry* __restrict__ bigq,
-                                                         const int* __restrict__ bigq_count, Ctl* ctl)
+                                                         int n, long long m, int* __restrict__ dist, SegQ q0,
+                                                         SegQ q1, const Entry* __restrict__ bigq,
+                                                         const int* __restrict__ bigq_count,
+                                                         unsigned* __restrict__ bm, int words, Ctl* ctl)
 {
   __shared__ int s_excl[kWavesPerBlock][kWave];
   __shared__ int s_beg[kWavesPerBlock][kWave];
@@ -362,35 +393,93 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   if (round_done(ctl)) return;
   const int iter = ctl->iter;
   const int mode = ctl->mode;
+  const int in_b = ctl->in_bits;
+  const bool out_q = (mode == 0) && (ctl->edges_next <= kQueueEdgesS);
   const int p = iter & 1;
-  const Entry* q_in = p ? q1 : q0;
-  Entry* q_out = p ? q0 : q1;
-  int* out_count = &ctl->wl_count[p ^ 1];
-  const int tag = iter + 1;
+  const SegQ q_in = p ? q1 : q0;
+  const SegQ q_out = p ? q0 : q1;
+  Entry* q_mine = segq_mine(q_out);
+  int* out_count = segq_counter(ctl, p ^ 1, my_group());
+  const unsigned* cur = bm + (size_t)(iter % 3) * words;
+  unsigned* next = bm + (size_t)((iter + 1) % 3) * words;
+  unsigned* stale = bm + (size_t)((iter + 2) % 3) * words;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const int gw = blockIdx.x * kWavesPerBlock + w;
   const int nwaves = gridDim.x * kWavesPerBlock;
-  long long pushed_edges = 0, work_e = 0, work_v = 0;
+  {
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < words; i += stride) stale[i] = 0u;
+  }
+  long long front_e = 0, scanned = 0, relaxed = 0, rows = 0;
+  // first marker of u in the next frontier: counts its edges, returns its row
+  auto mark = [&](int u, int& ub, int& ue) -> bool {
+    const unsigned bit = 1u << (u & 31);
+    if ((atomicOr(&next[u >> 5], bit) & bit) != 0) return false;
+    ub = nindex[u];
+    ue = nindex[u + 1];
+    front_e += ue - ub;
+    return true;
+  };
   if (mode == 0) {
-    const int qsize = ctl->wl_count[p];
-    if (blockIdx.x == 0 && threadIdx.x == 0) work_v = qsize;
-    wave_expand(q_in, qsize, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
-      work_e += 1;
+    auto relax = [&](const Entry& en, int e) {
+      scanned += 1;
+      relaxed += 1;
       const int nd = dist[en.v] + ew[e];
       const int u = nlist[e];
-      bool push = false;
+      bool first = false;
       int ub = 0, ue = 0;
-      if (dist[u] > nd && atomic_min_relaxed(&dist[u], nd) > nd) {
-        push = atomic_max_relaxed(&stamp[u], tag) != tag;
-        if (push) {
-          ub = nindex[u];
-          ue = nindex[u + 1];
-          pushed_edges += ue - ub;
+      if (dist[u] > nd && atomic_min_relaxed(&dist[u], nd) > nd) first = mark(u, ub, ue);
+      if (out_q) push_vertex_chunks(q_mine, out_count, first && ue > ub, u, ub, ue);
+    };
+    if (!in_b) {
+      const SegPrefix pre = segq_prefix(ctl, p);
+      if (blockIdx.x == 0 && threadIdx.x == 0) rows = pre.pre[kGroups];
+      wave_expand_with(pre.pre[kGroups], gw, nwaves, s_excl[w], s_ent[w],
+                       [&](int i) { return segq_at(q_in, pre, i); }, relax);
+    } else {
+      for (int base = gw * kWave; base < n; base += nwaves * kWave) {
+        const int v = base + lane;
+        const bool in = (v < n) && bit_of(cur, v);
+        if (ballot(in) == 0) continue;  // uniform
+        Entry ent = Entry{0, 0, 0, 0};
+        if (in) {
+          const int b = nindex[v], e = nindex[v + 1];
+          rows += 1;
+          if (e - b <= kChunk) ent = Entry{v, b, e, 0};
         }
+        const int len = ent.end - ent.beg;
+        const int incl = wave_incl_scan(len);
+        const int total = wave_bcast(incl, kWave - 1);
+        if (total == 0) continue;  // uniform
+        __builtin_amdgcn_wave_barrier();
+        s_excl[w][lane] = incl - len;
+        s_ent[w][lane] = ent;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (int tb = 0; tb < total; tb += kWave) {
+          const int t = tb + lane;
+          if (t < total) {
+            int lo = 0, hi = kWave;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_excl[w][mid] <= t) lo = mid;
+              else hi = mid;
+            }
+            const Entry& en = s_ent[w][lo];
+            relax(en, en.beg + (t - s_excl[w][lo]));
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
       }
-      push_vertex_chunks(q_out, out_count, push && ue > ub, u, ub, ue);
-    });
+      wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
+                       [&](int i) {
+                         Entry en = bigq[i];
+                         if (!bit_of(cur, en.v)) en.end = en.beg;
+                         return en;
+                       },
+                       relax);
+    }
   } else {
     int* excl = s_excl[w];
     int* rbeg = s_beg[w];
@@ -401,14 +490,12 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       if (v < n) {
         beg = nindex[v];
         len = nindex[v + 1] - beg;
+        rows += 1;
         if (len > kChunk) len = 0;
       }
       const int incl = wave_incl_scan(len);
       const int total = wave_bcast(incl, kWave - 1);
-      if (lane == 0) {
-        work_e += total;
-        work_v += (n - v0 < kWave) ? n - v0 : kWave;
-      }
+      if (lane == 0) scanned += total;
       excl[lane] = incl - len;
       rbeg[lane] = beg;
       rmin[lane] = kInf;
@@ -416,58 +503,82 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       for (int tb = 0; tb < total; tb += kWave) {
         const int t = tb + lane;
+        int lo = kWave, cand = kInf;  // lanes past the end: their own row kWave
         if (t < total) {
-          int lo = 0, hi = kWave;
+          int hi = kWave;
+          lo = 0;
           while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (excl[mid] <= t) lo = mid;
             else hi = mid;
           }
           const int e = rbeg[lo] + (t - excl[lo]);
-          const int du = dist[nlist[e]];
-          if (du != kInf) atomicMin(&rmin[lo], du + ew[e]);
+          const int u = nlist[e];
+          if (bit_of(cur, u)) {
+            relaxed += 1;
+            const int du = dist[u];
+            if (du != kInf) cand = du + ew[e];
+          }
         }
+        // rows occupy consecutive lanes: a segmented minimum over the wave
+        // leaves one LDS atomic per row segment instead of one per edge
+        #pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+          const int oc = __shfl_up(cand, d, kWave);
+          const int orow = __shfl_up(lo, d, kWave);
+          if (lane >= d && orow == lo) cand = min(cand, oc);
+        }
+        const int nrow = __shfl_down(lo, 1, kWave);
+        const bool tail = (lane == kWave - 1) || (nrow != lo);
+        if (tail && lo < kWave && cand != kInf) atomicMin(&rmin[lo], cand);
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      bool push = false;
-      if (len > 0 && rmin[lane] < dist[v]) {
+      const bool improved = len > 0 && rmin[lane] < dist[v];
+      if (improved) {
         dist[v] = rmin[lane];
-        push = true;
-        pushed_edges += len;
+        front_e += len;
       }
-      push_vertex_chunks(q_out, out_count, push, v, beg, beg + len);
+      // this wave owns the tile's bits except those of rows longer than
+      // kChunk, which the chunk walk below may set concurrently
+      const uint64_t mask = ballot(improved);
+      if (mask != 0) {
+        if (lane == 0 && (unsigned)mask != 0u) atomicOr(&next[v0 >> 5], (unsigned)mask);
+        if (lane == 1 && (unsigned)(mask >> 32) != 0u) atomicOr(&next[(v0 >> 5) + 1], (unsigned)(mask >> 32));
+      }
       __builtin_amdgcn_wave_barrier();
     }
-    const int nbig = *bigq_count;
-    wave_expand(bigq, nbig, gw, nwaves, s_excl[w], s_ent[w], [&](const Entry& en, int e) {
-      work_e += 1;
-      const int du = dist[nlist[e]];
-      const int cand = (du != kInf) ? du + ew[e] : kInf;
-      if (cand < dist[en.v] && atomic_min_relaxed(&dist[en.v], cand) > cand) {
-        if (atomic_max_relaxed(&stamp[en.v], tag) != tag) {
-          const int ub = nindex[en.v], ue = nindex[en.v + 1];
-          pushed_edges += ue - ub;
-          const int c = chunks_of(ue - ub);
-          const int base = atomicAdd(out_count, c);
-          for (int k = 0; k < c; k++) {
-            const int b = ub + k * kChunk;
-            q_out[base + k] = Entry{en.v, b, (b + kChunk < ue) ? b + kChunk : ue, 0};
-          }
-        }
-      }
-    });
+    wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w], [&](int i) { return bigq[i]; },
+                     [&](const Entry& en, int e) {
+                       scanned += 1;
+                       const int u = nlist[e];
+                       if (!bit_of(cur, u)) return;
+                       relaxed += 1;
+                       const int du = dist[u];
+                       const int cand = (du != kInf) ? du + ew[e] : kInf;
+                       if (cand < dist[en.v] && atomic_min_relaxed(&dist[en.v], cand) > cand) {
+                         int ub, ue;
+                         mark(en.v, ub, ue);
+                       }
+                     });
   }
-  block_add_ll2(&ctl->edges_next, pushed_edges, s_red);
-  block_add_ll2(&ctl->work_edges, work_e, s_red);
-  block_add_ll2(&ctl->work_vertices, work_v, s_red);
+  block_stat_add(ctl, kStatFrontierEdges, front_e, s_red);
+  block_stat_add(ctl, kStatScanned, scanned, s_red);
+  block_stat_add(ctl, kStatRelaxed, relaxed, s_red);
+  block_stat_add(ctl, kStatRows, rows, s_red);
   finish_round(ctl, [&] {
-    const int produced = load_relaxed(&ctl->wl_count[p ^ 1]);
-    const long long ef = load_relaxed(&ctl->edges_next);
-    ctl->wl_count[p] = 0;
-    ctl->edges_next = 0;
+    const long long ef = stat_take(ctl, kStatFrontierEdges);
+    ctl->work_edges += stat_take(ctl, kStatScanned);
+    ctl->aux_ll[0] += stat_take(ctl, kStatRelaxed);
+    ctl->work_vertices += stat_take(ctl, kStatRows);
+    // ef == 0: nothing was lowered, or only vertices without edges, from
+    // which nothing can propagate
+    GroupSlot* gs = ctl_groups(ctl);
+    for (int g = 0; g < kGroups; g++) gs[g].qcount[p] = 0;
     ctl->iter += 1;
-    if (produced == 0) ctl->done = 1;
+    ctl->in_bits = out_q ? 0 : 1;
+    ctl->edges_next = ef;
+    if (ef == 0) ctl->done = 1;
     else ctl->mode = (ef > m / kAlphaS) ? 1 : 0;
   });
 }
@@ -558,20 +669,26 @@ int style_edata_push(const igx_graph* g, int src, void* out, const void* aux, ig
   Runtime& R = rt();
   int* dist = (int*)out;
   const int* sp = (const int*)aux;
-  const size_t items = (size_t)std::max(g->edges, g->nodes);
-  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
+  // an edge is queued at most once per round (its source vertex's stamp), and
+  // in RW at most once more as a re-queued edge (its own stamp)
+  const size_t items = (size_t)std::max(g->edges, g->nodes) * (RW ? 2 : 1);
+  const size_t estamps = RW ? (size_t)std::max(g->edges, 1) : 1;
+  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
   IGX_TRY(ws_get(WS_A, sizeof(int) * items, &p0));
   IGX_TRY(ws_get(WS_B, sizeof(int) * items, &p1));
-  IGX_TRY(ws_get(WS_C, sizeof(int) * items, &p2));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * (size_t)g->nodes, &p2));
+  IGX_TRY(ws_get(WS_F, sizeof(int) * estamps, &p3));
   IGX_TRY(timer_start());
   IGX_TRY(ctl_reset(g->nodes));
   hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
-  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for((long long)items, kBlock)), dim3(kBlock), 0, R.stream, (int*)p2, 0,
-                     (long long)items);
+  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, (int*)p2, 0,
+                     (long long)g->nodes);
+  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for((long long)estamps, kBlock)), dim3(kBlock), 0, R.stream, (int*)p3, 0,
+                     (long long)estamps);
   hipLaunchKernelGGL(sssp_edata_seed, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, (int*)p0, R.d_ctl);
   return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
     hipLaunchKernelGGL((sssp_edata_push<RW>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
-                       g->eweight, sp, dist, (int*)p0, (int*)p1, (int*)p2, R.d_ctl);
+                       g->eweight, sp, dist, (int*)p0, (int*)p1, (int*)p2, (int*)p3, R.d_ctl);
   });
 }
 
@@ -602,29 +719,29 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
 {
   Runtime& R = rt();
   int* dist = (int*)out;
-  const size_t cap = (size_t)g->nodes + (size_t)g->edges / kChunk + 64;
-  void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr, *pb = nullptr, *pc = nullptr;
-  IGX_TRY(ws_get(WS_D, sizeof(Entry) * cap, &p0));
-  IGX_TRY(ws_get(WS_E, sizeof(Entry) * cap, &p1));
-  IGX_TRY(ws_get(WS_C, sizeof(int) * (size_t)g->nodes, &p2));
+  const long long cap = (long long)g->nodes + g->edges / kChunk + 64;  // chunks of all rows
+  const int words = ((g->nodes + kWave - 1) / kWave) * 2;
+  void *p0 = nullptr, *p1 = nullptr, *pm = nullptr, *pb = nullptr, *pc = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(Entry) * (size_t)cap * kGroups, &p0));
+  IGX_TRY(ws_get(WS_E, sizeof(Entry) * (size_t)cap * kGroups, &p1));
+  IGX_TRY(ws_get(WS_F, sizeof(unsigned) * 3 * (size_t)words, &pm));
+  // chunks of the rows longer than kChunk: at most 2 * edges / kChunk
   IGX_TRY(ws_get(WS_G, sizeof(Entry) * (2 * ((size_t)g->edges / kChunk) + 64), &pb));
   IGX_TRY(ws_get(WS_B, sizeof(int) * 64, &pc));
+  const SegQ q0{(Entry*)p0, cap}, q1{(Entry*)p1, cap};
   IGX_TRY(timer_start());
   IGX_TRY(ctl_reset(g->nodes));
   hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, dist, g->nodes);
-  hipLaunchKernelGGL(sssp_fill, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, (int*)p2, 0,
-                     (long long)g->nodes);
-  hipLaunchKernelGGL(sssp_fill, dim3(1), dim3(kBlock), 0, R.stream, (int*)pc, 0, 1LL);
-  hipLaunchKernelGGL(sssp_do_bigq, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nodes,
-                     (Entry*)pb, (int*)pc);
-  hipLaunchKernelGGL(sssp_bal_seed, dim3(1), dim3(kBlock), 0, R.stream, src, g->nindex, (Entry*)p0, R.d_ctl);
+  IGX_TRY(big_rows(g, pb, (int*)pc));
+  hipLaunchKernelGGL(sssp_do_init, dim3(blocks_for(2 * (long long)words, kBlock)), dim3(kBlock), 0, R.stream, src,
+                     g->nindex, (Entry*)p0, (unsigned*)pm, words, R.d_ctl);
   int rounds = 0, launches = 0;
   const int rc = run_rounds(
       [&](long long) {
         ktimer_begin("sssp_do_round");
         hipLaunchKernelGGL(sssp_do_round, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
-                           g->eweight, g->nodes, (long long)g->edges, dist, (Entry*)p0, (Entry*)p1, (int*)p2,
-                           (const Entry*)pb, (const int*)pc, R.d_ctl);
+                           g->eweight, g->nodes, (long long)g->edges, dist, q0, q1, (const Entry*)pb,
+                           (const int*)pc, (unsigned*)pm, words, R.d_ctl);
         ktimer_end("sssp_do_round");
       },
       sssp_round_cap(g), &rounds, &launches, 8);
@@ -632,6 +749,7 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
   info->iterations = rounds;
   info->launches = launches;
   info->edges_processed = (unsigned long long)R.h_ctl->work_edges;
+  info->edges_relaxed = (unsigned long long)R.h_ctl->aux_ll[0];
   info->vertices_processed = (unsigned long long)R.h_ctl->work_vertices;
   return rc;
 }

@@ -54,6 +54,11 @@ def style_id(algo: str, style: Union[int, str, None]) -> int:
     return names.index(style)
 
 
+def default_entry_style(algo: str, edge_kind: int = 0) -> int:
+    """Style index used by the vertex (0) or edge (1) reference-signature entry."""
+    return lib().igx_get_entry_style(ALGOS[algo], 1 if edge_kind else 0)
+
+
 def set_pr_params(eps: float, max_iter: int) -> None:
     check(lib().igx_set_pr_params(eps, max_iter), "igx_set_pr_params")
 
