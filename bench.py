@@ -99,6 +99,51 @@ def algorithmic_bytes(algo, n, m):
             "mis": 12 * m + 12 * n + 4, "pr": 8 * m + 16 * n + 4, "tc": 4 * m + 4 * n + 4}[algo]
 
 
+def pr_tc_section(args, D, G):
+    """PageRank (pull, d = 0.85, stop at L1 residual < 1e-6 or 100 iterations)
+    and triangle counting on R-MAT scale 20 and on a 2048 x 2048 grid where 5%
+    of the vertices gain one random long-range edge."""
+    out = {}
+    graphs = (("rmat20", G.rmat(20, 16, seed=1)), ("grid2048", G.grid2d(2048, 0.05, seed=1)))
+    D.set_pr_params(1e-6, 100)
+    try:
+        for tag, g in graphs:
+            n, m = g.nodes, g.edges
+            dg = D.DeviceGraph(g)
+            buf = D.DeviceBuffer(max(4 * n, 8))
+            for algo in ("pr", "tc"):
+                style = D.styles(algo)[D.style_id(algo, None)]
+                infos = [D.run_device(algo, dg, buf, style, 0) for _ in range(3)]
+                med = float(np.median([i.seconds for i in infos]))
+                it = infos[-1].iterations if algo == "pr" else 1
+                b = algorithmic_bytes(algo, n, m) * max(it, 1)
+                entry = {"style": style, "n": n, "m_directed": m, "seconds": round(med, 6),
+                         "gteps": round(m * max(it, 1) / med * 1e-9, 3), "hbm_frac": round(b / med / (HBM_PEAK_GBS * 1e9), 4)}
+                if algo == "pr":
+                    entry["iterations"] = it
+                    entry["residual"] = infos[-1].residual
+                    if not args.no_verify:
+                        import oracle as O
+                        ref, _, _ = O.pr_fp64(g, fixed_iters=it)
+                        got = buf.to_host(np.float32, n).astype(np.float64)
+                        entry["max_rel_err_vs_fp64"] = float(np.max(np.abs(got - ref) / np.abs(ref)))
+                else:
+                    entry["triangles"] = int(infos[-1].count)
+                    if not args.no_verify:
+                        # cross-check with a style of another algorithm (sorted-list merge per
+                        # edge vs the default); the oracle's count takes a minute at this size
+                        other = "e_bsearch_thread" if style != "e_bsearch_thread" else "v_merge_thread"
+                        alt = D.run_device("tc", dg, buf, other, 0).count
+                        assert alt == entry["triangles"], (alt, entry["triangles"])
+                        entry["cross_checked_with"] = other
+                out[f"{algo}_{tag}"] = entry
+            buf.free()
+            dg.free()
+    finally:
+        D.set_pr_params(1e-4, 100)
+    return out
+
+
 def cpu_baseline(scale):
     import oracle as O
     from indigo2suite_amd import graph as G
@@ -122,6 +167,7 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip PageRank and TC on R-MAT 20 and the grid")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -197,6 +243,20 @@ def main():
         per_algo[a] = {"style": styles[a], "seconds": round(med, 6), "gteps": round(m / med * 1e-9, 3),
                        "hbm_frac": round(b / med / (HBM_PEAK_GBS * 1e9), 4)}
     per_algo["sssp"]["iterations"] = iters["sssp"]
+
+    # MIS on the same graph
+    mis_buf = D.DeviceBuffer(4 * n)
+    mis_style = D.styles("mis")[D.style_id("mis", None)]
+    mis_t = [D.run_device("mis", dg, mis_buf, mis_style, 0).seconds for _ in range(args.steps)]
+    med = float(np.median(mis_t))
+    per_algo["mis"] = {"style": mis_style, "seconds": round(med, 6), "gteps": round(m / med * 1e-9, 3),
+                       "hbm_frac": round(algorithmic_bytes("mis", n, m) / med / (HBM_PEAK_GBS * 1e9), 4)}
+    if not args.no_verify:
+        import oracle as O
+        assert O.mis_verify(g, mis_buf.to_host(np.int32, n).astype(np.uint8)) == 0
+    mis_buf.free()
+    if not args.no_extra:
+        per_algo.update(pr_tc_section(args, D, G))
 
     # the configuration's named SSSP style (edge-based, data-driven worklist)
     named = D.run_device("sssp", dg, bufs["sssp"], "e_data_push_thread", src)

@@ -395,11 +395,9 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
       wave_expand_with(pre.pre[kGroups], gw, nwaves, s_excl[w], s_ent[w],
                        [&](int i) { return segq_at(q_in, pre, i); }, discover);
     } else {
-      // rows of at most kChunk edges, 64-vertex tiles of the bitmap
-      for (int base = gw * kWave; base < n; base += nwaves * kWave) {
-        const int u = base + lane;
-        const bool in = (u < n) && in_bits(cur, u);
-        if (ballot(in) == 0) continue;  // uniform
+      // rows of at most kChunk edges, the non-empty 64-vertex tiles of the bitmap
+      for_each_bitmap_tile(cur, n, gw, nwaves, [&](int v0, bool in) {
+        const int u = v0 + lane;
         Entry ent = Entry{0, 0, 0, 0};
         if (in) {
           const int b = nindex[u], e = nindex[u + 1];
@@ -408,7 +406,7 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
         const int len = ent.end - ent.beg;
         const int incl = wave_incl_scan(len);
         const int total = wave_bcast(incl, kWave - 1);
-        if (total == 0) continue;  // uniform
+        if (total == 0) return;  // uniform
         __builtin_amdgcn_wave_barrier();
         s_excl[w][lane] = incl - len;
         s_ent[w][lane] = ent;
@@ -428,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
           }
         }
         __builtin_amdgcn_wave_barrier();
-      }
+      });
       // longer rows: their precomputed chunks, kept when the row is in the frontier
       wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
                        [&](int i) {
@@ -439,46 +437,70 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
                        discover);
     }
   } else {
+    // kSub 64-vertex sub-tiles per step, their loads issued together (the
+    // chain dist -> row -> first neighbours -> frontier bits is latency-bound)
     constexpr int kFirst = 4;
-    for (int base = gw * kWave; base < n; base += nwaves * kWave) {
-      const int u = base + lane;
-      bool unvisited = false, found = false;
-      int ub = 0, ue = 0;
-      if (u < n && dist[u] == kInf) {
-        unvisited = true;
-        ub = nindex[u];
-        ue = nindex[u + 1];
-        int nb[kFirst];
-        #pragma unroll
-        for (int k = 0; k < kFirst; k++) nb[k] = (ub + k < ue) ? nlist[ub + k] : -1;
-        #pragma unroll
-        for (int k = 0; k < kFirst; k++) found = found || (nb[k] >= 0 && in_bits(cur, nb[k]));
+    constexpr int kSub = 4;
+    for (int base = gw * kWave * kSub; base < n; base += nwaves * kWave * kSub) {
+      bool unvisited[kSub], found[kSub];
+      int ub[kSub], ue[kSub];
+      #pragma unroll
+      for (int s = 0; s < kSub; s++) {
+        const int u = base + s * kWave + lane;
+        unvisited[s] = (u < n) && dist[u] == kInf;
+        found[s] = false;
+        ub[s] = ue[s] = 0;
       }
-      uint64_t pending = ballot(unvisited && !found && ue - ub > kFirst);
-      while (pending != 0) {
-        const int L = __ffsll((unsigned long long)pending) - 1;
-        pending &= pending - 1;
-        const int s0 = __shfl(ub, L, kWave) + kFirst;
-        const int e0 = __shfl(ue, L, kWave);
-        bool hit = false;
-        for (int i0 = s0; i0 < e0; i0 += kWave) {
-          const int i = i0 + lane;
-          const bool h = (i < e0) && in_bits(cur, nlist[i]);
-          if (ballot(h) != 0) {
-            hit = true;
-            break;
-          }
+      #pragma unroll
+      for (int s = 0; s < kSub; s++) {
+        const int u = base + s * kWave + lane;
+        if (unvisited[s]) {
+          ub[s] = nindex[u];
+          ue[s] = nindex[u + 1];
         }
-        if (lane == L) found = hit;
       }
-      if (found) {
-        dist[u] = nd;
-        found_v += 1;
-        found_e += ue - ub;
+      int nb[kSub][kFirst];
+      #pragma unroll
+      for (int s = 0; s < kSub; s++) {
+        #pragma unroll
+        for (int k = 0; k < kFirst; k++) nb[s][k] = (unvisited[s] && ub[s] + k < ue[s]) ? nlist[ub[s] + k] : -1;
       }
-      const uint64_t mask = ballot(found);
-      if (lane == 0) next[base >> 5] = (unsigned)mask;
-      if (lane == 1) next[(base >> 5) + 1] = (unsigned)(mask >> 32);
+      #pragma unroll
+      for (int s = 0; s < kSub; s++) {
+        #pragma unroll
+        for (int k = 0; k < kFirst; k++) found[s] = found[s] || (nb[s][k] >= 0 && in_bits(cur, nb[s][k]));
+      }
+      #pragma unroll
+      for (int s = 0; s < kSub; s++) {
+        if (base + s * kWave >= n) break;  // uniform
+        uint64_t pending = ballot(unvisited[s] && !found[s] && ue[s] - ub[s] > kFirst);
+        while (pending != 0) {
+          const int L = __ffsll((unsigned long long)pending) - 1;
+          pending &= pending - 1;
+          const int s0 = __shfl(ub[s], L, kWave) + kFirst;
+          const int e0 = __shfl(ue[s], L, kWave);
+          bool hit = false;
+          for (int i0 = s0; i0 < e0; i0 += kWave) {
+            const int i = i0 + lane;
+            const bool h = (i < e0) && in_bits(cur, nlist[i]);
+            if (ballot(h) != 0) {
+              hit = true;
+              break;
+            }
+          }
+          if (lane == L) found[s] = hit;
+        }
+        const int u = base + s * kWave + lane;
+        if (found[s]) {
+          dist[u] = nd;
+          found_v += 1;
+          found_e += ue[s] - ub[s];
+        }
+        const uint64_t mask = ballot(found[s]);
+        const int wd = (base + s * kWave) >> 5;
+        if (lane == 0) next[wd] = (unsigned)mask;
+        if (lane == 1) next[wd + 1] = (unsigned)(mask >> 32);
+      }
     }
   }
   block_stat_add(ctl, 0, found_v, s_red);

@@ -359,14 +359,14 @@ __global__ void __launch_bounds__(kBlock) uf_compress(int n, int* __restrict__ p
 
 // -------------------------------------------------- sampled union-find ----
 // Link every vertex to its first kAfRounds neighbours, compress, find the
-// root shared by most of 1024 sampled vertices (the giant component), then
+// root shared by most of kAfSamples sampled vertices (the giant component), then
 // hook the remaining edges of only those vertices outside it.  An edge with
 // an endpoint outside the sampled component is processed from that endpoint;
 // an edge with both endpoints inside joins nothing new.  Skipping is decided
 // by reading parent[v] == c, which only ever holds for vertices already
 // connected to c.  The root of every tree is still its minimum id.
 constexpr int kAfRounds = 2;
-constexpr int kAfSamples = 1024;
+constexpr int kAfSamples = 256;
 
 // parent[v] = v's first neighbour when that is smaller: linking a vertex to a
 // smaller vertex needs no atomics while every vertex is still a root, and the

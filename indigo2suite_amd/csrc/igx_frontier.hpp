@@ -105,6 +105,33 @@ __device__ __forceinline__ Entry segq_at(const SegQ& q, const SegPrefix& sp, int
   return q.base[(long long)s * q.cap + (idx - sp.pre[s])];
 }
 
+// Visit every 64-vertex tile [v0, v0 + 64) that holds at least one set bit of
+// the bitmap, one wave per tile: body(v0, bit) with `bit` this lane's vertex
+// v0 + lane.  A wave loads 64 words (2048 vertices) at once and skips empty
+// tiles without further loads, so a sparse frontier costs one coalesced pass
+// over the bitmap.  All lanes of the wave call this together; bits at or past
+// n must be clear.
+template <typename F>
+__device__ __forceinline__ void for_each_bitmap_tile(const unsigned* __restrict__ bits, int n, int gw, int nwaves,
+                                                     F body)
+{
+  const int lane = lane_id();
+  const int nwords = (n + 31) >> 5;
+  for (int wb = gw * kWave; wb < nwords; wb += nwaves * kWave) {  // uniform
+    const int wi = wb + lane;
+    const unsigned word = (wi < nwords) ? bits[wi] : 0u;
+    const uint64_t nz = ballot(word != 0u);
+    uint64_t pair = (nz | (nz >> 1)) & 0x5555555555555555ull;  // even j: word j or j + 1 non-empty
+    while (pair != 0) {
+      const int j = __ffsll((unsigned long long)pair) - 1;
+      pair &= pair - 1;
+      const unsigned w0 = __shfl(word, j, kWave), w1 = __shfl(word, j + 1, kWave);
+      const unsigned mine = (lane < 32) ? w0 : w1;
+      body((wb + j) * 32, ((mine >> (lane & 31)) & 1u) != 0u);
+    }
+  }
+}
+
 // Append the edge ids [beg, beg + len) of every lane to an edge worklist:
 // one atomic per wave reserves the wave's total, then the 64 lanes write the
 // concatenated ranges together (each lane finds its range by binary search

@@ -185,6 +185,9 @@ __global__ void __launch_bounds__(kBlock) big_rows_write(const int* __restrict__
     tot += s_wave2[k];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *count = tot;
+  __shared__ int s_pos[kBlock];
+  __shared__ int s_beg[kBlock];
+  __shared__ int s_end[kBlock];
   int v0, v1;
   big_rows_span(n, &v0, &v1);
   for (int b0 = v0; b0 < v1; b0 += kBlock) {  // uniform per block
@@ -204,12 +207,25 @@ __global__ void __launch_bounds__(kBlock) big_rows_write(const int* __restrict__
       if (k < w) woff += s_wave[k];
       btot += s_wave[k];
     }
-    const int pos = run + woff + incl - c;
-    for (int k = 0; k < c; k++) {
-      const int b = beg + k * kChunk;
-      bigq[pos + k] = Entry{v, b, (b + kChunk < end) ? b + kChunk : end, 0};
+    s_pos[threadIdx.x] = woff + incl - c;
+    s_beg[threadIdx.x] = beg;
+    s_end[threadIdx.x] = end;
+    __syncthreads();
+    // the block writes the tile's chunks together: chunk j belongs to the
+    // last row k with s_pos[k] <= j (rows without chunks share the next offset)
+    for (int j = threadIdx.x; j < btot; j += kBlock) {
+      int lo = 0, hi = kBlock;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pos[mid] <= j) lo = mid;
+        else hi = mid;
+      }
+      const int b = s_beg[lo] + (j - s_pos[lo]) * kChunk;
+      const int e = s_end[lo];
+      bigq[run + j] = Entry{b0 + lo, b, (b + kChunk < e) ? b + kChunk : e, 0};
     }
     run += btot;
+    __syncthreads();
   }
 }
 

@@ -348,6 +348,7 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
 // rows visited (8 B offsets + 4 B distance).
 constexpr int kAlphaS = 8;
 constexpr long long kQueueEdgesS = 1 << 16;
+constexpr int kSlice = 512;  // edges per streamed slice of a pull run
 
 enum { kStatFrontierEdges = 0, kStatScanned = 1, kStatRelaxed = 2, kStatRows = 3 };
 
@@ -386,8 +387,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
                                                          unsigned* __restrict__ bm, int words, Ctl* ctl)
 {
   __shared__ int s_excl[kWavesPerBlock][kWave];
-  __shared__ int s_beg[kWavesPerBlock][kWave];
-  __shared__ int s_min[kWavesPerBlock][kWave];
+  __shared__ int s_cand[kWavesPerBlock][kSlice];
   __shared__ Entry s_ent[kWavesPerBlock][kWave];
   __shared__ long long s_red[kWavesPerBlock];
   if (round_done(ctl)) return;
@@ -438,10 +438,8 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       wave_expand_with(pre.pre[kGroups], gw, nwaves, s_excl[w], s_ent[w],
                        [&](int i) { return segq_at(q_in, pre, i); }, relax);
     } else {
-      for (int base = gw * kWave; base < n; base += nwaves * kWave) {
-        const int v = base + lane;
-        const bool in = (v < n) && bit_of(cur, v);
-        if (ballot(in) == 0) continue;  // uniform
+      for_each_bitmap_tile(cur, n, gw, nwaves, [&](int v0, bool in) {
+        const int v = v0 + lane;
         Entry ent = Entry{0, 0, 0, 0};
         if (in) {
           const int b = nindex[v], e = nindex[v + 1];
@@ -451,7 +449,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         const int len = ent.end - ent.beg;
         const int incl = wave_incl_scan(len);
         const int total = wave_bcast(incl, kWave - 1);
-        if (total == 0) continue;  // uniform
+        if (total == 0) return;  // uniform
         __builtin_amdgcn_wave_barrier();
         s_excl[w][lane] = incl - len;
         s_ent[w][lane] = ent;
@@ -471,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
           }
         }
         __builtin_amdgcn_wave_barrier();
-      }
+      });
       wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
                        [&](int i) {
                          Entry en = bigq[i];
@@ -481,62 +479,70 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
                        relax);
     }
   } else {
-    int* excl = s_excl[w];
-    int* rbeg = s_beg[w];
-    int* rmin = s_min[w];
+    // 64-row tiles.  Consecutive rows of at most kChunk edges form runs whose
+    // edges are contiguous in nlist; a run is streamed in slices of kSlice
+    // edges: the wave loads kSlice / 64 edges per lane with independent loads
+    // (ids, frontier bits, distances, weights), parks the candidates in LDS,
+    // and each lane then folds its own row's part of the slice.
+    int* cbuf = s_cand[w];
     for (int v0 = gw * kWave; v0 < n; v0 += nwaves * kWave) {
       const int v = v0 + lane;
-      int beg = 0, len = 0;
+      int rb = 0, re = 0;
       if (v < n) {
-        beg = nindex[v];
-        len = nindex[v + 1] - beg;
+        rb = nindex[v];
+        re = nindex[v + 1];
         rows += 1;
-        if (len > kChunk) len = 0;
       }
-      const int incl = wave_incl_scan(len);
-      const int total = wave_bcast(incl, kWave - 1);
-      if (lane == 0) scanned += total;
-      excl[lane] = incl - len;
-      rbeg[lane] = beg;
-      rmin[lane] = kInf;
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      for (int tb = 0; tb < total; tb += kWave) {
-        const int t = tb + lane;
-        int lo = kWave, cand = kInf;  // lanes past the end: their own row kWave
-        if (t < total) {
-          int hi = kWave;
-          lo = 0;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (excl[mid] <= t) lo = mid;
-            else hi = mid;
+      const bool small = (v < n) && (re - rb <= kChunk);
+      int acc = kInf;
+      uint64_t todo = ballot(small);
+      while (todo != 0) {  // uniform: one run of consecutive small rows
+        const int first = __ffsll((unsigned long long)todo) - 1;
+        const uint64_t from = todo >> first;
+        const int cnt = (~from == 0) ? (kWave - first) : (__ffsll((unsigned long long)~from) - 1);
+        const int last = first + cnt - 1;
+        todo &= (cnt + first >= kWave) ? ((1ull << first) - 1) : ~(((1ull << cnt) - 1) << first);
+        const int r0 = __shfl(rb, first, kWave), r1 = __shfl(re, last, kWave);
+        const bool mine = lane >= first && lane <= last;
+        if (lane == 0) scanned += r1 - r0;
+        for (int c0 = r0; c0 < r1; c0 += kSlice) {
+          const int c1 = (r1 - c0 < kSlice) ? r1 : c0 + kSlice;
+          int u[kSlice / kWave];
+          #pragma unroll
+          for (int k = 0; k < kSlice / kWave; k++) {
+            const int e = c0 + k * kWave + lane;
+            u[k] = (e < c1) ? nlist[e] : -1;
           }
-          const int e = rbeg[lo] + (t - excl[lo]);
-          const int u = nlist[e];
-          if (bit_of(cur, u)) {
-            relaxed += 1;
-            const int du = dist[u];
-            if (du != kInf) cand = du + ew[e];
+          bool hit[kSlice / kWave];
+          #pragma unroll
+          for (int k = 0; k < kSlice / kWave; k++) hit[k] = u[k] >= 0 && bit_of(cur, u[k]);
+          int du[kSlice / kWave], wt[kSlice / kWave];
+          #pragma unroll
+          for (int k = 0; k < kSlice / kWave; k++) {
+            const int e = c0 + k * kWave + lane;
+            du[k] = hit[k] ? dist[u[k]] : kInf;
+            wt[k] = hit[k] ? ew[e] : 0;
           }
+          __builtin_amdgcn_wave_barrier();
+          #pragma unroll
+          for (int k = 0; k < kSlice / kWave; k++) {
+            relaxed += hit[k] ? 1 : 0;
+            cbuf[k * kWave + lane] = (du[k] != kInf) ? du[k] + wt[k] : kInf;
+          }
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          if (mine) {
+            const int lo = (rb > c0) ? rb : c0;
+            const int hi = (re < c1) ? re : c1;
+            for (int i = lo; i < hi; i++) acc = min(acc, cbuf[i - c0]);
+          }
+          __builtin_amdgcn_wave_barrier();
         }
-        // rows occupy consecutive lanes: a segmented minimum over the wave
-        // leaves one LDS atomic per row segment instead of one per edge
-        #pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-          const int oc = __shfl_up(cand, d, kWave);
-          const int orow = __shfl_up(lo, d, kWave);
-          if (lane >= d && orow == lo) cand = min(cand, oc);
-        }
-        const int nrow = __shfl_down(lo, 1, kWave);
-        const bool tail = (lane == kWave - 1) || (nrow != lo);
-        if (tail && lo < kWave && cand != kInf) atomicMin(&rmin[lo], cand);
       }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      const bool improved = len > 0 && rmin[lane] < dist[v];
+      const int len = re - rb;
+      const bool improved = small && len > 0 && acc < dist[v];
       if (improved) {
-        dist[v] = rmin[lane];
+        dist[v] = acc;
         front_e += len;
       }
       // this wave owns the tile's bits except those of rows longer than
@@ -546,21 +552,65 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         if (lane == 0 && (unsigned)mask != 0u) atomicOr(&next[v0 >> 5], (unsigned)mask);
         if (lane == 1 && (unsigned)(mask >> 32) != 0u) atomicOr(&next[(v0 >> 5) + 1], (unsigned)(mask >> 32));
       }
-      __builtin_amdgcn_wave_barrier();
     }
-    wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w], [&](int i) { return bigq[i]; },
-                     [&](const Entry& en, int e) {
-                       scanned += 1;
-                       const int u = nlist[e];
-                       if (!bit_of(cur, u)) return;
-                       relaxed += 1;
-                       const int du = dist[u];
-                       const int cand = (du != kInf) ? du + ew[e] : kInf;
-                       if (cand < dist[en.v] && atomic_min_relaxed(&dist[en.v], cand) > cand) {
-                         int ub, ue;
-                         mark(en.v, ub, ue);
-                       }
-                     });
+    // rows longer than kChunk: one wave per chunk, two chunks per step, their
+    // 2 x kChunk edges loaded together, a wave minimum per chunk, and one
+    // atomicMin on each row's distance
+    const int nbig = *bigq_count;
+    constexpr int kPer = kChunk / kWave;
+    constexpr int kPair = 2;
+    for (int c0 = gw; c0 < nbig; c0 += kPair * nwaves) {  // uniform per wave
+      Entry en[kPair];
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {
+        const int ci = c0 + h * nwaves;
+        en[h] = (ci < nbig) ? bigq[ci] : Entry{0, 0, 0, 0};
+      }
+      int u[kPair][kPer];
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {
+        #pragma unroll
+        for (int k = 0; k < kPer; k++) {
+          const int e = en[h].beg + k * kWave + lane;
+          u[h][k] = (e < en[h].end) ? nlist[e] : -1;
+        }
+      }
+      bool hit[kPair][kPer];
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {
+        #pragma unroll
+        for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && bit_of(cur, u[h][k]);
+      }
+      int cand[kPair];
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {
+        cand[h] = kInf;
+        #pragma unroll
+        for (int k = 0; k < kPer; k++) {
+          if (hit[h][k]) {
+            relaxed += 1;
+            const int du = dist[u[h][k]];
+            if (du != kInf) cand[h] = min(cand[h], du + ew[en[h].beg + k * kWave + lane]);
+          }
+        }
+      }
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {
+        #pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) cand[h] = min(cand[h], __shfl_xor(cand[h], d, kWave));
+      }
+      if (lane == 0) {
+        #pragma unroll
+        for (int h = 0; h < kPair; h++) {
+          scanned += en[h].end - en[h].beg;
+          const int v = en[h].v;
+          if (en[h].end > en[h].beg && cand[h] < dist[v] && atomic_min_relaxed(&dist[v], cand[h]) > cand[h]) {
+            int ub, ue;
+            mark(v, ub, ue);
+          }
+        }
+      }
+    }
   }
   block_stat_add(ctl, kStatFrontierEdges, front_e, s_red);
   block_stat_add(ctl, kStatScanned, scanned, s_red);
