@@ -130,12 +130,8 @@ def pr_tc_section(args, D, G):
                 else:
                     entry["triangles"] = int(infos[-1].count)
                     if not args.no_verify:
-                        # cross-check with a style of another algorithm (sorted-list merge per
-                        # edge vs the default); the oracle's count takes a minute at this size
-                        other = "e_bsearch_thread" if style != "e_bsearch_thread" else "v_merge_thread"
-                        alt = D.run_device("tc", dg, buf, other, 0).count
-                        assert alt == entry["triangles"], (alt, entry["triangles"])
-                        entry["cross_checked_with"] = other
+                        import oracle as O
+                        assert O.tc_forward(g) == entry["triangles"]
                 out[f"{algo}_{tag}"] = entry
             buf.free()
             dg.free()

@@ -35,16 +35,6 @@ __global__ void __launch_bounds__(kBlock) fill_int(int* __restrict__ p, int valu
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) p[i] = value;
 }
 
-__global__ void finalize_flag_kernel(Ctl* ctl)
-{
-  if (threadIdx.x == 0 && !round_done(ctl)) finalize_flag_round(ctl);
-}
-
-__global__ void finalize_worklist_kernel(Ctl* ctl)
-{
-  if (threadIdx.x == 0 && !round_done(ctl)) finalize_worklist_round(ctl);
-}
-
 __device__ __forceinline__ void raise_flag(Ctl* ctl, bool updated)
 {
   const uint64_t any = ballot(updated);
@@ -95,7 +85,6 @@ __global__ void __launch_bounds__(kBlock) bfs_vtopo_push(const int* __restrict__
     }
   }
   raise_flag(ctl, updated);
-  if constexpr (PERSIST) finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 template <int G>
@@ -138,7 +127,6 @@ __global__ void __launch_bounds__(kBlock) bfs_vtopo_pull(const int* __restrict__
     }
   }
   raise_flag(ctl, updated);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 // ------------------------------------------------------- edge, topology ----
@@ -158,7 +146,6 @@ __global__ void __launch_bounds__(kBlock) bfs_etopo_push(const int* __restrict__
     }
   }
   raise_flag(ctl, updated);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 // ------------------------------------------------------- vertex, data ----
@@ -227,7 +214,6 @@ __global__ void __launch_bounds__(kBlock) bfs_vdata_push(const int* __restrict__
       }
     }
   }
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // --------------------------------------------------------- edge, data ----
@@ -282,7 +268,6 @@ __global__ void __launch_bounds__(kBlock) bfs_edata_push(const int* __restrict__
     }
     wave_push_edge_ranges(wl_out, out_count, rbeg, rlen, s_excl[w], s_beg[w]);
   }
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // ------------------------------------------ direction-optimizing hybrid ----
@@ -347,7 +332,7 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
 {
   __shared__ int s_excl[kWavesPerBlock][kWave];
   __shared__ Entry s_ent[kWavesPerBlock][kWave];
-  __shared__ long long s_red[kWavesPerBlock];
+  __shared__ long long s_red[kWavesPerBlock * kStats];
   if (round_done(ctl)) return;
   const int level = ctl->level;
   const int mode = ctl->mode;
@@ -503,11 +488,18 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
       }
     }
   }
-  block_stat_add(ctl, 0, found_v, s_red);
-  block_stat_add(ctl, 1, found_e, s_red);
-  finish_round(ctl, [&] {
-    const long long nf = stat_take(ctl, 0);
-    const long long mf = stat_take(ctl, 1);
+  round_partials(ctl, found_v, found_e, 0, 0, s_red);
+}
+
+// Level decision (run by round_finalize after each bfs_do_level): totals s[0]
+// = vertices found, s[1] = their edges.
+struct FinBfsDo {
+  __device__ void operator()(Ctl* ctl, const RoundTotals& t) const
+  {
+    const long long nf = t.s[0];
+    const long long mf = t.s[1];
+    const int p = ctl->level & 1;
+    const bool out_b = (ctl->mode == 1) || (ctl->edges_next > kQueueEdges);
     ctl->edges_unvisited -= mf;
     if (nf == 0) {
       ctl->done = 1;
@@ -522,8 +514,8 @@ __global__ void __launch_bounds__(kBlock) bfs_do_level(const int* __restrict__ n
     for (int g = 0; g < kGroups; g++) gs[g].qcount[p] = 0;
     ctl->level += 1;
     ctl->iter += 1;
-  });
-}
+  }
+};
 
 // ------------------------------------------------------------ host side ----
 struct Bufs {
@@ -534,7 +526,7 @@ struct Bufs {
 
 template <typename K>
 int run_topo(const igx_graph* g, int src, int* d_dist, igx_run_info* info, int grid, long long max_rounds,
-             K kernel_launch, bool need_finalize)
+             K kernel_launch)
 {
   Runtime& R = rt();
   IGX_TRY(timer_start());
@@ -544,7 +536,7 @@ int run_topo(const igx_graph* g, int src, int* d_dist, igx_run_info* info, int g
   const int rc = run_rounds(
       [&](long long) {
         kernel_launch(grid);
-        if (need_finalize) hipLaunchKernelGGL(finalize_flag_kernel, dim3(1), dim3(64), 0, R.stream, R.d_ctl);
+        launch_fin_flag();
       },
       max_rounds, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
@@ -567,8 +559,7 @@ int style_vtopo_push(const igx_graph* g, int src, void* out, const void*, igx_ru
                   [&](int gr) {
                     hipLaunchKernelGGL((bfs_vtopo_push<G, AK, DETERM, PERSIST>), dim3(gr), dim3(kBlock), 0, R.stream,
                                        g->nindex, g->nlist, g->nodes, dist, R.d_ctl);
-                  },
-                  !PERSIST);
+                  });
 }
 
 template <int G>
@@ -580,8 +571,7 @@ int style_vtopo_pull(const igx_graph* g, int src, void* out, const void*, igx_ru
                   [&](int gr) {
                     hipLaunchKernelGGL((bfs_vtopo_pull<G>), dim3(gr), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                                        g->nodes, dist, R.d_ctl);
-                  },
-                  false);
+                  });
 }
 
 template <int AK>
@@ -594,8 +584,7 @@ int style_etopo_push(const igx_graph* g, int src, void* out, const void* aux, ig
                   [&](int gr) {
                     hipLaunchKernelGGL((bfs_etopo_push<AK>), dim3(gr), dim3(kBlock), 0, R.stream, g->nlist, sp,
                                        g->edges, dist, R.d_ctl);
-                  },
-                  false);
+                  });
 }
 
 int get_worklists(size_t items, size_t stamp_items, Bufs* b)
@@ -629,6 +618,7 @@ int style_vdata_push(const igx_graph* g, int src, void* out, const void*, igx_ru
       [&](long long) {
         hipLaunchKernelGGL((bfs_vdata_push<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
                            g->nlist, dist, b.wl0, b.wl1, b.stamp, R.d_ctl);
+        launch_fin_worklist();
       },
       (long long)g->nodes + 2, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
@@ -656,6 +646,7 @@ int style_edata_push(const igx_graph* g, int src, void* out, const void* aux, ig
       [&](long long) {
         hipLaunchKernelGGL(bfs_edata_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                            sp, dist, b.wl0, b.wl1, b.stamp, R.d_ctl);
+        launch_fin_worklist();
       },
       (long long)g->nodes + 2, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
@@ -691,6 +682,7 @@ int style_do_hybrid(const igx_graph* g, int src, void* out, const void*, igx_run
         hipLaunchKernelGGL(bfs_do_level, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                            g->nodes, dist, q0, q1, (const Entry*)pq, (const int*)pc, (unsigned*)pb, words, R.d_ctl);
         ktimer_end("bfs_do_level");
+        launch_finalize(R.persist_blocks, 0x3, FinBfsDo{});
       },
       (long long)g->nodes + 2, &rounds, &launches, 8);
   IGX_TRY(timer_stop(&info->seconds));

@@ -34,10 +34,6 @@ __global__ void __launch_bounds__(kBlock) cc_fill(int* __restrict__ p, int value
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) p[i] = value;
 }
 
-__global__ void cc_finalize_flag_kernel(Ctl* ctl)
-{
-  if (threadIdx.x == 0 && !round_done(ctl)) finalize_flag_round(ctl);
-}
 
 __device__ __forceinline__ void cc_raise_flag(Ctl* ctl, bool updated)
 {
@@ -81,7 +77,6 @@ __global__ void __launch_bounds__(kBlock) cc_vtopo_push(const int* __restrict__ 
     }
   }
   cc_raise_flag(ctl, updated);
-  if constexpr (PERSIST) finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 __global__ void __launch_bounds__(kBlock) cc_vtopo_pull(const int* __restrict__ nindex,
@@ -101,7 +96,6 @@ __global__ void __launch_bounds__(kBlock) cc_vtopo_pull(const int* __restrict__ 
     }
   }
   cc_raise_flag(ctl, updated);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 __global__ void __launch_bounds__(kBlock) cc_etopo_push(const int* __restrict__ nlist, const int* __restrict__ sp,
@@ -116,7 +110,6 @@ __global__ void __launch_bounds__(kBlock) cc_etopo_push(const int* __restrict__ 
     if (label[dst] > lv && atomic_min_relaxed(&label[dst], lv) > lv) updated = true;
   }
   cc_raise_flag(ctl, updated);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 // worklist seeded with all items [0, count)
@@ -157,7 +150,6 @@ __global__ void __launch_bounds__(kBlock) cc_vdata_push(const int* __restrict__ 
       if (push) wl_out[slot] = dst;
     }
   }
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // One thread per worklist edge; when dst's label drops, all of dst's edges
@@ -195,7 +187,6 @@ __global__ void __launch_bounds__(kBlock) cc_edata_push(const int* __restrict__ 
     }
     wave_push_edge_ranges(wl_out, out_count, rbeg, rlen, s_excl[w], s_beg[w]);
   }
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // ------------------------------------------------------------- union-find --
@@ -523,7 +514,7 @@ __global__ void __launch_bounds__(kBlock) af_link_rest(const int* __restrict__ n
 
 // ------------------------------------------------------------ host side ----
 template <typename K>
-int run_cc_topo(const igx_graph* g, int* d_label, igx_run_info* info, K launch, bool need_finalize)
+int run_cc_topo(const igx_graph* g, int* d_label, igx_run_info* info, K launch)
 {
   Runtime& R = rt();
   IGX_TRY(timer_start());
@@ -533,7 +524,7 @@ int run_cc_topo(const igx_graph* g, int* d_label, igx_run_info* info, K launch, 
   const int rc = run_rounds(
       [&](long long) {
         launch();
-        if (need_finalize) hipLaunchKernelGGL(cc_finalize_flag_kernel, dim3(1), dim3(64), 0, R.stream, R.d_ctl);
+        launch_fin_flag();
       },
       (long long)g->nodes + 2, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
@@ -555,7 +546,7 @@ int style_vtopo_push(const igx_graph* g, int, void* out, const void*, igx_run_in
   return run_cc_topo(g, label, info, [&] {
     hipLaunchKernelGGL((cc_vtopo_push<G, AK, PERSIST>), dim3(grid), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        g->nodes, label, R.d_ctl);
-  }, !PERSIST);
+  });
 }
 
 int style_vtopo_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
@@ -565,7 +556,7 @@ int style_vtopo_pull(const igx_graph* g, int, void* out, const void*, igx_run_in
   return run_cc_topo(g, label, info, [&] {
     hipLaunchKernelGGL(cc_vtopo_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        g->nodes, label, R.d_ctl);
-  }, false);
+  });
 }
 
 int style_etopo_push(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
@@ -576,7 +567,7 @@ int style_etopo_push(const igx_graph* g, int, void* out, const void* aux, igx_ru
   return run_cc_topo(g, label, info, [&] {
     hipLaunchKernelGGL(cc_etopo_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, sp, g->edges,
                        label, R.d_ctl);
-  }, false);
+  });
 }
 
 template <bool EDGE>
@@ -612,6 +603,7 @@ int style_data_push(const igx_graph* g, int, void* out, const void* aux, igx_run
           hipLaunchKernelGGL(cc_vdata_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                              label, wl0, wl1, stamp, R.d_ctl);
         }
+        launch_fin_worklist();
       },
       (long long)g->nodes + 2, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));

@@ -153,7 +153,7 @@ struct Ctl {
   int done;              // 1 once converged
   int iter;              // rounds completed
   int flag;              // "goagain" of the current round
-  unsigned arrived;      // groups finished in the current round
+  unsigned pad0;
   int wl_count[2];       // worklist sizes, indexed by round parity
   int mode;              // algorithm-specific (BFS: 0 top-down, 1 bottom-up)
   int level;             // BFS level being expanded
@@ -168,86 +168,84 @@ struct Ctl {
   int in_bits;           // BFS: the current frontier is held by its bitmap only
 };
 
-// Same-address atomics from thousands of workgroups serialise (one every
-// ~15 ns on MI355X), so a round's arrivals, statistics and queue counters are
-// spread over kGroups slots, each on its own 4 KiB page: workgroup b uses slot
-// b % kGroups, which is also the XCD that the hardware deals it to.  The slots
-// follow the control block in the same allocation.
+// Round epilogue.  Ending a round inside the round kernel ("the last block to
+// arrive finalises") needs a device-scope fence and an atomic per block; on
+// MI355X the fences alone cost ~30 us per 2048-block launch (each one writes
+// back the XCD's L2), and same-address atomics serialise at ~11 ns each.  So a
+// round is two launches instead: the round kernel leaves per-block partial
+// statistics with plain stores (no fence: the kernel boundary publishes them),
+// and a one-workgroup finalize kernel reduces them in block order
+// (deterministic, also for the FP64 residual) and updates the control block.
+//
+// Queue appends still need counters: those are spread over kGroups slots on
+// their own 4 KiB pages (workgroup b appends to slot b % kGroups, which is also
+// the XCD the hardware deals it to).  Slots and partials follow the control
+// block in the same allocation.
 constexpr int kGroups = 8;
 constexpr int kCtlBytes = 4096;
-constexpr int kStats = 4;
+constexpr int kStats = 4;        // integer partials per block
+constexpr int kStatsD = 2;       // FP64 partials per block
+constexpr int kMaxParts = 8192;  // largest grid of a round kernel
 
 struct alignas(4096) GroupSlot {
-  unsigned arrived;            // workgroups of this group finished in the round
-  unsigned pad0[31];
-  long long stat[kStats];      // per-round partial sums (reset by the finaliser)
-  long long pad1[12];
-  int qcount[2];               // queue segment sizes, indexed by round parity
-  int pad2[30];
+  int qcount[2];  // queue segment sizes, indexed by round parity
+  int pad[1022];
 };
 static_assert(sizeof(Ctl) <= kCtlBytes, "control block exceeds its page");
-constexpr size_t kCtlAllocBytes = kCtlBytes + kGroups * sizeof(GroupSlot);
+constexpr size_t kPartsOffset = kCtlBytes + kGroups * sizeof(GroupSlot);
+constexpr size_t kCtlAllocBytes = kPartsOffset + (size_t)(kStats + kStatsD) * kMaxParts * 8;
 
 __device__ __forceinline__ GroupSlot* ctl_groups(Ctl* ctl)
 {
   return reinterpret_cast<GroupSlot*>(reinterpret_cast<char*>(ctl) + kCtlBytes);
 }
+__device__ __forceinline__ long long* ctl_parts(Ctl* ctl)
+{
+  return reinterpret_cast<long long*>(reinterpret_cast<char*>(ctl) + kPartsOffset);
+}
+__device__ __forceinline__ double* ctl_parts_d(Ctl* ctl)
+{
+  return reinterpret_cast<double*>(ctl_parts(ctl) + (size_t)kStats * kMaxParts);
+}
 __device__ __forceinline__ int my_group() { return (int)(blockIdx.x % kGroups); }
 
-// Block-level epilogue of one round.  All threads of the block must call it.
-// Arrival is two-level: a block counts in its group's slot; the last block of
-// a group counts in ctl->arrived; `finalize` runs on exactly one thread of the
-// last block overall, after device-scope fences, and must reset per-round
-// fields it uses (group stats: stat_take).
-template <typename F>
-__device__ __forceinline__ void finish_round(Ctl* ctl, F finalize)
+// Round kernel side: every block of a launch that did not exit on `done` calls
+// this once, with all threads, and stores its sums of up to kStats values.
+// lds: kWavesPerBlock * kStats slots.
+__device__ __forceinline__ void round_partials(Ctl* ctl, long long a, long long b, long long c, long long d,
+                                               long long* lds)
 {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned nb = gridDim.x;
-    const unsigned grp = blockIdx.x % kGroups;
-    const unsigned ngroups = nb < (unsigned)kGroups ? nb : (unsigned)kGroups;
-    const unsigned gsize = (nb - 1 - grp) / kGroups + 1;
-    GroupSlot* gs = ctl_groups(ctl) + grp;
-    __threadfence();
-    if (atomicAdd(&gs->arrived, 1u) == gsize - 1) {
-      gs->arrived = 0;  // every block of the group has arrived: reusable next round
-      __threadfence();
-      if (atomicAdd(&ctl->arrived, 1u) == ngroups - 1) {
-        __threadfence();
-        finalize();
-        ctl->arrived = 0;
-        __threadfence();
-      }
-    }
-  }
-}
-
-// Add a block's value to statistic k of its group (one atomic per block).
-__device__ __forceinline__ void block_stat_add(Ctl* ctl, int k, long long v, long long* lds4)
-{
-  v = wave_sum_ll(v);
+  a = wave_sum_ll(a);
+  b = wave_sum_ll(b);
+  c = wave_sum_ll(c);
+  d = wave_sum_ll(d);
   const int w = threadIdx.x / kWave;
-  if (lane_id() == 0) lds4[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long t = 0;
-    for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += lds4[i];  // lds4: one slot per wave
-    if (t != 0) atomicAdd((unsigned long long*)&ctl_groups(ctl)[my_group()].stat[k], (unsigned long long)t);
+  if (lane_id() == 0) {
+    lds[w * kStats + 0] = a;
+    lds[w * kStats + 1] = b;
+    lds[w * kStats + 2] = c;
+    lds[w * kStats + 3] = d;
   }
   __syncthreads();
+  if ((int)threadIdx.x < kStats) {
+    long long t = 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += lds[i * kStats + threadIdx.x];
+    ctl_parts(ctl)[(size_t)threadIdx.x * kMaxParts + blockIdx.x] = t;
+  }
 }
 
-// Finaliser side: the round's total of statistic k, clearing the partials.
-__device__ __forceinline__ long long stat_take(Ctl* ctl, int k)
+// FP64 partial `slot` of this block (all threads call; lds: kWavesPerBlock).
+__device__ __forceinline__ void round_partial_d(Ctl* ctl, int slot, double v, double* lds)
 {
-  GroupSlot* gs = ctl_groups(ctl);
-  long long t = 0;
-  for (int g = 0; g < kGroups; g++) {
-    t += load_relaxed(&gs[g].stat[k]);
-    gs[g].stat[k] = 0;
+  v = wave_sum_d(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += lds[i];
+    ctl_parts_d(ctl)[(size_t)slot * kMaxParts + blockIdx.x] = t;
   }
-  return t;
 }
 
 // the common topology-driven rule: stop after a round without updates
@@ -273,6 +271,70 @@ __device__ __forceinline__ bool round_done(const Ctl* ctl)
 {
   return *(volatile const int*)&ctl->done != 0;
 }
+
+struct RoundTotals {
+  long long s[kStats];
+  double d[kStatsD];
+};
+
+constexpr int kFinBlock = 256;
+
+// Finalize kernel (one workgroup of kFinBlock threads): sums the partials of
+// the nparts blocks of the round kernel for the statistics in `stat_mask`
+// (bit k: integer k; bit kStats + j: FP64 j), then runs fin(ctl, totals) on
+// one thread.  Does nothing once the run is done.
+template <typename Fin>
+__global__ void __launch_bounds__(kFinBlock) round_finalize(Ctl* ctl, int nparts, int stat_mask, Fin fin)
+{
+  __shared__ long long s_l[kFinBlock];
+  __shared__ double s_d[kFinBlock];
+  __shared__ RoundTotals tot;
+  if (round_done(ctl)) return;
+  for (int k = 0; k < kStats + kStatsD; k++) {
+    if (!((stat_mask >> k) & 1)) {
+      if (threadIdx.x == 0) {
+        if (k < kStats) tot.s[k] = 0;
+        else tot.d[k - kStats] = 0.0;
+      }
+      continue;  // uniform
+    }
+    if (k < kStats) {
+      const long long* p = ctl_parts(ctl) + (size_t)k * kMaxParts;
+      long long t = 0;
+      for (int b = threadIdx.x; b < nparts; b += kFinBlock) t += p[b];
+      s_l[threadIdx.x] = t;
+    } else {
+      const double* p = ctl_parts_d(ctl) + (size_t)(k - kStats) * kMaxParts;
+      double t = 0.0;
+      for (int b = threadIdx.x; b < nparts; b += kFinBlock) t += p[b];
+      s_d[threadIdx.x] = t;
+    }
+    __syncthreads();
+    for (int off = kFinBlock / 2; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) {
+        if (k < kStats) s_l[threadIdx.x] += s_l[threadIdx.x + off];
+        else s_d[threadIdx.x] += s_d[threadIdx.x + off];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (k < kStats) tot.s[k] = s_l[0];
+      else tot.d[k - kStats] = s_d[0];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) fin(ctl, tot);
+}
+
+
+// finalizers of the two common round rules
+struct FinFlag {
+  __device__ void operator()(Ctl* ctl, const RoundTotals&) const { finalize_flag_round(ctl); }
+};
+struct FinWorklist {
+  __device__ void operator()(Ctl* ctl, const RoundTotals&) const { finalize_worklist_round(ctl); }
+};
 
 // MIS priority: the reference's integer hash of (v + 712313887), compared as
 // unsigned (mis_vertex_omp.h:127-132 and codes/omp MIS init).

@@ -86,6 +86,19 @@ int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches, 
 // Clear the control block on the stream (kernel, not a memset node).
 int ctl_reset(int n);
 
+// End of a round, enqueued after the round kernel: the topology-driven rule
+// (stop after a round without updates), the data-driven rule (stop after a
+// round that appended nothing), or a reducing finalizer over the round
+// kernel's per-block partials (round_finalize in igx_device.hpp).
+void launch_fin_flag();
+void launch_fin_worklist();
+template <typename Fin>
+inline void launch_finalize(int nparts, int stat_mask, Fin fin)
+{
+  hipLaunchKernelGGL((round_finalize<Fin>), dim3(1), dim3(kFinBlock), 0, rt().stream, rt().d_ctl, nparts, stat_mask,
+                     fin);
+}
+
 inline int blocks_for(long long items, int per_block)
 {
   long long b = (items + per_block - 1) / per_block;

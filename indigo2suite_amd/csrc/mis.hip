@@ -11,6 +11,8 @@
 //   v_topo_pull_*  V_Topo_Pull siblings (exclude self when a neighbour is in)
 //   v_data_pull_*  V_Data_Pull siblings: worklist of undecided vertices
 //   e_topo_thread  [C-MET]:57-127 / [O-MISe]:52-94 edge pass + vertex pass
+//   v_resume_pull  (no reference counterpart) pull over the outranking
+//                  neighbours only, resuming where the last round stopped
 #include "igx_internal.hpp"
 
 namespace igx {
@@ -92,7 +94,6 @@ __global__ void __launch_bounds__(kBlock) mis_vtopo_push(const int* __restrict__
     }
   }
   mis_raise_flag(ctl, pending);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 // pull: a vertex excludes itself when a neighbour is in the set and includes
@@ -126,7 +127,6 @@ __global__ void __launch_bounds__(kBlock) mis_vtopo_pull(const int* __restrict__
     else store_relaxed(&status[v], d);
   }
   mis_raise_flag(ctl, pending);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 __global__ void __launch_bounds__(kBlock) mis_seed(int* __restrict__ wl, int n, Ctl* ctl)
@@ -165,7 +165,6 @@ __global__ void __launch_bounds__(kBlock) mis_vdata_pull(const int* __restrict__
     const int slot = wave_push_slot(out_count, keep);
     if (keep) wl_out[slot] = v;
   }
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // [O-MISe]:52-77 edge pass: exclude the neighbours of included vertices and
@@ -209,15 +208,114 @@ __global__ void __launch_bounds__(kBlock) mis_vertex_pass(int n, int* __restrict
     }
   }
   mis_raise_flag(ctl, pending);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
+}
+
+// ------------------------------------------------------- resumable pull ----
+// Only the neighbours that outrank v matter: if one of them is in the set v is
+// out; if all of them are out v is in; a lower-ranked neighbour cannot be in
+// the set while v is undecided.  So an undecided vertex walks its outranking
+// neighbours in list order and stops at the first one not yet excluded: in
+// the set -> v is excluded; undecided -> v waits, and remembers where it
+// stopped (pos[v]: every outranking neighbour before it is excluded, which is
+// final), so a later round resumes there instead of rescanning the row.
+// Each lane walks at most kMisLane neighbours on its own per round; lanes with
+// more to walk are finished by the whole wave, 64 neighbours per step.  The
+// fixed point is the same greedy set; only the round in which a vertex is
+// decided can differ from the full-scan styles.
+constexpr int kMisLane = 32;
+
+__global__ void __launch_bounds__(kBlock) mis_pos_init(const int* __restrict__ nindex, int* __restrict__ pos, int n)
+{
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) pos[v] = nindex[v];
+}
+
+__global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict__ nindex,
+                                                           const int* __restrict__ nlist, int n,
+                                                           int* __restrict__ status, int* __restrict__ pos, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int lane = lane_id();
+  const int gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  bool pending = false;
+  for (int base = gw * kWave; base < n; base += nwaves * kWave) {  // uniform per wave
+    const int v = base + lane;
+    const bool undec = (v < n) && load_relaxed(&status[v]) == kUndecided;
+    const uint32_t pv = mis_priority((uint32_t)v);
+    int i = 0, end = 0, result = kUndecided;
+    bool more = false;
+    if (undec) {
+      i = pos[v];
+      end = nindex[v + 1];
+      int steps = 0;
+      for (; i < end && steps < kMisLane; i++, steps++) {
+        const int u = nlist[i];
+        if (u == v || !outranks(mis_priority((uint32_t)u), u, pv, v)) continue;
+        const int su = load_relaxed(&status[u]);
+        if (su == kIncluded) {
+          result = kExcluded;
+          break;
+        }
+        if (su == kUndecided) break;  // waits for u; resumes here
+      }
+      if (result == kUndecided && i >= end) result = kIncluded;
+      more = result == kUndecided && i < end && steps >= kMisLane;
+    }
+    uint64_t todo = ballot(more);
+    while (todo != 0) {
+      const int L = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const int vL = base + L;
+      const uint32_t pvL = mis_priority((uint32_t)vL);
+      const int s0 = __shfl(i, L, kWave), e0 = __shfl(end, L, kWave);
+      int res = kIncluded, stop = e0;
+      for (int i0 = s0; i0 < e0; i0 += kWave) {
+        const int idx = i0 + lane;
+        int su = kExcluded;
+        if (idx < e0) {
+          const int u = nlist[idx];
+          if (u != vL && outranks(mis_priority((uint32_t)u), u, pvL, vL)) su = load_relaxed(&status[u]);
+        }
+        const uint64_t hit = ballot(su != kExcluded);
+        if (hit != 0) {
+          const int first = __ffsll((unsigned long long)hit) - 1;
+          const int sf = __shfl(su, first, kWave);
+          res = (sf == kIncluded) ? kExcluded : kUndecided;
+          stop = i0 + first;
+          break;
+        }
+      }
+      if (lane == L) {
+        result = res;
+        i = stop;
+      }
+    }
+    if (undec) {
+      if (result != kUndecided) {
+        store_relaxed(&status[v], result);
+      } else {
+        pos[v] = i;
+        pending = true;
+      }
+    }
+  }
+  mis_raise_flag(ctl, pending);
 }
 
 // ------------------------------------------------------------ host side ----
+// worklist: the round ends by the data-driven rule, else by the flag rule
 template <typename K>
-int mis_rounds(const igx_graph* g, igx_run_info* info, K launch)
+int mis_rounds(const igx_graph* g, igx_run_info* info, K launch, bool worklist = false)
 {
   int rounds = 0, launches = 0;
-  const int rc = run_rounds([&](long long) { launch(); }, (long long)g->nodes + 2, &rounds, &launches);
+  const int rc = run_rounds(
+      [&](long long) {
+        launch();
+        if (worklist) launch_fin_worklist();
+        else launch_fin_flag();
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
   info->iterations = rounds;
   info->launches = launches;
@@ -268,6 +366,25 @@ int style_vdata_pull(const igx_graph* g, int, void* out, const void*, igx_run_in
     hipLaunchKernelGGL(mis_vdata_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        status, (int*)p0, (int*)p1, R.d_ctl);
     ktimer_end("mis_vdata_pull");
+  }, true);
+}
+
+int style_resume_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* status = (int*)out;
+  void* pp = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)g->nodes, &pp));
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, status, g->nodes);
+  hipLaunchKernelGGL(mis_pos_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, g->nindex,
+                     (int*)pp, g->nodes);
+  return mis_rounds(g, info, [&] {
+    ktimer_begin("mis_resume_pull");
+    hipLaunchKernelGGL(mis_resume_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+                       g->nodes, status, (int*)pp, R.d_ctl);
+    ktimer_end("mis_resume_pull");
   });
 }
 
@@ -297,6 +414,7 @@ const StyleEntry kStyles[] = {
     {"v_topo_pull_thread", style_vtopo_pull, 0},
     {"v_data_pull_thread", style_vdata_pull, 0},
     {"e_topo_thread", style_etopo, 1},
+    {"v_resume_pull", style_resume_pull, 0},
 };
 
 }  // namespace

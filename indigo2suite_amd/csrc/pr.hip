@@ -88,28 +88,21 @@ __device__ __forceinline__ double pr_finish(int v, Pair acc, float base, const i
   return fabs((double)value - (double)old);
 }
 
-__device__ __forceinline__ void block_add_d(double* target, double v, double* lds)
-{
-  v = wave_sum_d(v);
-  const int w = threadIdx.x / kWave;
-  if (lane_id() == 0) lds[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0;
-    for (int k = 0; k < kWavesPerBlock; k++) t += lds[k];
-    if (t != 0.0) atomicAdd(target, t);
-  }
-  __syncthreads();
-}
 
-__device__ __forceinline__ void pr_finalize_round(Ctl* ctl, double eps, int max_iter)
-{
-  const double r = load_relaxed(&ctl->res);
-  ctl->res_last = r;
-  ctl->res = 0.0;
-  ctl->iter += 1;
-  if (r < eps || ctl->iter >= max_iter) ctl->done = 1;
-}
+// End of an iteration (run by round_finalize): the FP64 L1 residual is the
+// block-ordered sum of the per-block partials (slot 0: rows scored by the main
+// kernel, slot 1: rows merged from chunks), so it is the same on every run.
+struct FinPr {
+  double eps;
+  int max_iter;
+  __device__ void operator()(Ctl* ctl, const RoundTotals& t) const
+  {
+    const double r = t.d[0] + t.d[1];
+    ctl->res_last = r;
+    ctl->iter += 1;
+    if (r < eps || ctl->iter >= max_iter) ctl->done = 1;
+  }
+};
 
 enum Gran { kThread = 0, kWaveG = 1, kBlockG = 2 };
 
@@ -156,8 +149,7 @@ __global__ void __launch_bounds__(kBlock) pr_pull(const int* __restrict__ nindex
       __syncthreads();
     }
   }
-  block_add_d(&ctl->res, err, s_red);
-  finish_round(ctl, [&] { pr_finalize_round(ctl, eps, max_iter); });
+  round_partial_d(ctl, 0, err, s_red);
 }
 
 // ------------------------------------------------------------- balanced ----
@@ -247,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) pr_bal_main(const int* __restrict__ ni
     acc = wave_merge(acc);
     if (lane == 0) partial[k] = acc;
   }
-  block_add_d(&ctl->res, err, s_red);
+  round_partial_d(ctl, 0, err, s_red);
 }
 
 __global__ void __launch_bounds__(kBlock) pr_bal_big(const int* __restrict__ nindex, float base,
@@ -267,8 +259,7 @@ __global__ void __launch_bounds__(kBlock) pr_bal_big(const int* __restrict__ nin
     for (int k = 0; k < b.count; k++) acc = pair_merge(acc, partial[b.first + k]);
     err += pr_finish(b.v, acc, base, nindex, score, contrib_next);
   }
-  block_add_d(&ctl->res, err, s_red);
-  finish_round(ctl, [&] { pr_finalize_round(ctl, eps, max_iter); });
+  round_partial_d(ctl, 1, err, s_red);
 }
 
 // ------------------------------------------------------------ host side ----
@@ -302,6 +293,7 @@ int style_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* in
         float* nxt = (r & 1) ? c0 : c1;
         hipLaunchKernelGGL((pr_pull<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                            g->nodes, base, score, cur, nxt, R.d_ctl, P.eps, P.max_iter);
+        launch_finalize(R.persist_blocks, 1 << kStats, FinPr{P.eps, P.max_iter});
       },
       P.max_iter, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
@@ -334,7 +326,6 @@ int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info
   hipLaunchKernelGGL(pr_init, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, R.stream, g->nindex, n, score, c0);
   hipLaunchKernelGGL(pr_classify, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, n, (int*)pmed,
                      (int*)pcnt, (Entry*)pch, (BigRow*)pbig);
-  const int big_grid = blocks_for((long long)nbig, kBlock);
   int rounds = 0, launches = 0;
   const int rc = run_rounds(
       [&](long long r) {
@@ -345,8 +336,10 @@ int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info
                            base, score, cur, nxt, (const int*)pmed, (const int*)pcnt, (const Entry*)pch,
                            (Pair*)ppart, R.d_ctl);
         ktimer_end("pr_bal_main");
-        hipLaunchKernelGGL(pr_bal_big, dim3(big_grid), dim3(kBlock), 0, R.stream, g->nindex, base, score, nxt,
-                           (const BigRow*)pbig, (const int*)pcnt, (const Pair*)ppart, R.d_ctl, P.eps, P.max_iter);
+        hipLaunchKernelGGL(pr_bal_big, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, base, score,
+                           nxt, (const BigRow*)pbig, (const int*)pcnt, (const Pair*)ppart, R.d_ctl, P.eps,
+                           P.max_iter);
+        launch_finalize(R.persist_blocks, 3 << kStats, FinPr{P.eps, P.max_iter});
       },
       P.max_iter, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));

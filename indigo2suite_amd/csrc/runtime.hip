@@ -40,11 +40,19 @@ __global__ void ctl_reset_kernel(Ctl* ctl, int n)
   }
   if ((int)threadIdx.x < kGroups) {
     GroupSlot* gs = ctl_groups(ctl) + threadIdx.x;
-    gs->arrived = 0;
-    for (int k = 0; k < kStats; k++) gs->stat[k] = 0;
     gs->qcount[0] = 0;
     gs->qcount[1] = 0;
   }
+}
+
+__global__ void fin_flag_kernel(Ctl* ctl)
+{
+  if (threadIdx.x == 0 && !round_done(ctl)) finalize_flag_round(ctl);
+}
+
+__global__ void fin_worklist_kernel(Ctl* ctl)
+{
+  if (threadIdx.x == 0 && !round_done(ctl)) finalize_worklist_round(ctl);
 }
 
 __global__ void build_sp_kernel(const int* __restrict__ nindex, int nodes, int* __restrict__ sp)
@@ -97,6 +105,9 @@ int ws_get(int slot, size_t bytes, void** ptr)
   *ptr = g_ws[slot];
   return IGX_OK;
 }
+
+void launch_fin_flag() { hipLaunchKernelGGL(fin_flag_kernel, dim3(1), dim3(64), 0, g_rt.stream, g_rt.d_ctl); }
+void launch_fin_worklist() { hipLaunchKernelGGL(fin_worklist_kernel, dim3(1), dim3(64), 0, g_rt.stream, g_rt.d_ctl); }
 
 int ctl_reset(int n)
 {
@@ -308,11 +319,11 @@ static void init_entry_styles()
   g_entry_style[IGX_SSSP][1] = find_style(IGX_SSSP, "e_data_push_thread");
   g_entry_style[IGX_CC][0] = find_style(IGX_CC, "uf_afforest");
   g_entry_style[IGX_CC][1] = find_style(IGX_CC, "e_topo_push_thread");
-  g_entry_style[IGX_MIS][0] = find_style(IGX_MIS, "v_data_pull_thread");
+  g_entry_style[IGX_MIS][0] = find_style(IGX_MIS, "v_resume_pull");
   g_entry_style[IGX_MIS][1] = find_style(IGX_MIS, "e_topo_thread");
   g_entry_style[IGX_PR][0] = find_style(IGX_PR, "pull_balanced_kahan");
   g_entry_style[IGX_PR][1] = g_entry_style[IGX_PR][0];
-  g_entry_style[IGX_TC][0] = find_style(IGX_TC, "v_merge_wave");
+  g_entry_style[IGX_TC][0] = find_style(IGX_TC, "oriented_lds");
   g_entry_style[IGX_TC][1] = find_style(IGX_TC, "e_bsearch_thread");
   g_entry_style_set = true;
 }
@@ -340,6 +351,7 @@ int igx_init(int device)
   g_rt.max_threads_per_cu = prop.maxThreadsPerMultiProcessor;
   g_rt.persist_blocks = g_rt.cus * (g_rt.max_threads_per_cu / kBlock);
   if (g_rt.persist_blocks < 1) g_rt.persist_blocks = 1;
+  if (g_rt.persist_blocks > kMaxParts) g_rt.persist_blocks = kMaxParts;  // one partial slot per block
   IGX_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
   IGX_HIP(hipEventCreate(&g_rt.ev_start));
   IGX_HIP(hipEventCreate(&g_rt.ev_stop));

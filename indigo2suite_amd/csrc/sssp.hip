@@ -84,7 +84,6 @@ __global__ void __launch_bounds__(kBlock) sssp_vtopo_push(const int* __restrict_
     }
   }
   sssp_raise_flag(ctl, updated);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 __global__ void __launch_bounds__(kBlock) sssp_vtopo_pull(const int* __restrict__ nindex,
@@ -108,7 +107,6 @@ __global__ void __launch_bounds__(kBlock) sssp_vtopo_pull(const int* __restrict_
     }
   }
   sssp_raise_flag(ctl, updated);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 __global__ void __launch_bounds__(kBlock) sssp_etopo_push(const int* __restrict__ nlist,
@@ -127,7 +125,6 @@ __global__ void __launch_bounds__(kBlock) sssp_etopo_push(const int* __restrict_
     }
   }
   sssp_raise_flag(ctl, updated);
-  finish_round(ctl, [&] { finalize_flag_round(ctl); });
 }
 
 // ------------------------------------------------------- vertex, data ----
@@ -193,7 +190,6 @@ __global__ void __launch_bounds__(kBlock) sssp_vdata_push(const int* __restrict_
       }
     }
   }
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // --------------------------------------------------------- edge, data ----
@@ -266,7 +262,6 @@ __global__ void __launch_bounds__(kBlock) sssp_edata_push(const int* __restrict_
     }
     wave_push_edge_ranges(wl_out, out_count, rbeg, rlen, s_excl[w], s_beg[w]);
   }
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // --------------------------------------------- balanced vertex worklist ----
@@ -319,7 +314,6 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
     }
     push_vertex_chunks(q_out, out_count, push && ue > ub, u, ub, ue);
   });
-  finish_round(ctl, [&] { finalize_worklist_round(ctl); });
 }
 
 // ------------------------------------------ direction-optimizing rounds ----
@@ -389,7 +383,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   __shared__ int s_excl[kWavesPerBlock][kWave];
   __shared__ int s_cand[kWavesPerBlock][kSlice];
   __shared__ Entry s_ent[kWavesPerBlock][kWave];
-  __shared__ long long s_red[kWavesPerBlock];
+  __shared__ long long s_red[kWavesPerBlock * kStats];
   if (round_done(ctl)) return;
   const int iter = ctl->iter;
   const int mode = ctl->mode;
@@ -612,33 +606,44 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       }
     }
   }
-  block_stat_add(ctl, kStatFrontierEdges, front_e, s_red);
-  block_stat_add(ctl, kStatScanned, scanned, s_red);
-  block_stat_add(ctl, kStatRelaxed, relaxed, s_red);
-  block_stat_add(ctl, kStatRows, rows, s_red);
-  finish_round(ctl, [&] {
-    const long long ef = stat_take(ctl, kStatFrontierEdges);
-    ctl->work_edges += stat_take(ctl, kStatScanned);
-    ctl->aux_ll[0] += stat_take(ctl, kStatRelaxed);
-    ctl->work_vertices += stat_take(ctl, kStatRows);
-    // ef == 0: nothing was lowered, or only vertices without edges, from
-    // which nothing can propagate
+  round_partials(ctl, front_e, scanned, relaxed, rows, s_red);
+}
+
+// Round decision (run by round_finalize after each sssp_do_round).
+struct FinSsspDo {
+  long long m;
+  __device__ void operator()(Ctl* ctl, const RoundTotals& t) const
+  {
+    const long long ef = t.s[kStatFrontierEdges];
+    const bool out_q = (ctl->mode == 0) && (ctl->edges_next <= kQueueEdgesS);
+    const int p = ctl->iter & 1;
+    ctl->work_edges += t.s[kStatScanned];
+    ctl->aux_ll[0] += t.s[kStatRelaxed];
+    ctl->work_vertices += t.s[kStatRows];
     GroupSlot* gs = ctl_groups(ctl);
     for (int g = 0; g < kGroups; g++) gs[g].qcount[p] = 0;
     ctl->iter += 1;
     ctl->in_bits = out_q ? 0 : 1;
     ctl->edges_next = ef;
+    // ef == 0: nothing was lowered, or only vertices without edges, from
+    // which nothing can propagate
     if (ef == 0) ctl->done = 1;
     else ctl->mode = (ef > m / kAlphaS) ? 1 : 0;
-  });
-}
+  }
+};
 
 // ------------------------------------------------------------ host side ----
 template <typename K>
-int run_sssp_rounds(const igx_graph* g, igx_run_info* info, long long max_rounds, K launch)
+int run_sssp_rounds(const igx_graph* g, igx_run_info* info, long long max_rounds, K launch, bool worklist)
 {
   int rounds = 0, launches = 0;
-  const int rc = run_rounds([&](long long) { launch(); }, max_rounds, &rounds, &launches);
+  const int rc = run_rounds(
+      [&](long long) {
+        launch();
+        if (worklist) launch_fin_worklist();
+        else launch_fin_flag();
+      },
+      max_rounds, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
   info->iterations = rounds;
   info->launches = launches;
@@ -661,7 +666,7 @@ int style_vtopo_push(const igx_graph* g, int src, void* out, const void*, igx_ru
   return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
     hipLaunchKernelGGL((sssp_vtopo_push<G, AK>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
                        g->nlist, g->eweight, g->nodes, dist, R.d_ctl);
-  });
+  }, false);
 }
 
 int style_vtopo_pull(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
@@ -674,7 +679,7 @@ int style_vtopo_pull(const igx_graph* g, int src, void* out, const void*, igx_ru
   return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
     hipLaunchKernelGGL(sssp_vtopo_pull, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        g->eweight, g->nodes, dist, R.d_ctl);
-  });
+  }, false);
 }
 
 int style_etopo_push(const igx_graph* g, int src, void* out, const void* aux, igx_run_info* info)
@@ -688,7 +693,7 @@ int style_etopo_push(const igx_graph* g, int src, void* out, const void* aux, ig
   return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
     hipLaunchKernelGGL(sssp_etopo_push, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, g->eweight, sp,
                        g->edges, dist, R.d_ctl);
-  });
+  }, false);
 }
 
 template <int G>
@@ -710,7 +715,7 @@ int style_vdata_push(const igx_graph* g, int src, void* out, const void*, igx_ru
   return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
     hipLaunchKernelGGL((sssp_vdata_push<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        g->eweight, dist, (int*)p0, (int*)p1, (int*)p2, R.d_ctl);
-  });
+  }, true);
 }
 
 template <bool RW>
@@ -739,7 +744,7 @@ int style_edata_push(const igx_graph* g, int src, void* out, const void* aux, ig
   return run_sssp_rounds(g, info, sssp_round_cap(g), [&] {
     hipLaunchKernelGGL((sssp_edata_push<RW>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        g->eweight, sp, dist, (int*)p0, (int*)p1, (int*)p2, (int*)p3, R.d_ctl);
-  });
+  }, true);
 }
 
 int style_balanced(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
@@ -762,7 +767,7 @@ int style_balanced(const igx_graph* g, int src, void* out, const void*, igx_run_
     hipLaunchKernelGGL(sssp_bal_round, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                        g->eweight, dist, (Entry*)p0, (Entry*)p1, (int*)p2, R.d_ctl);
     ktimer_end("sssp_bal_round");
-  });
+  }, true);
 }
 
 int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
@@ -793,6 +798,7 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
                            g->eweight, g->nodes, (long long)g->edges, dist, q0, q1, (const Entry*)pb,
                            (const int*)pc, (unsigned*)pm, words, R.d_ctl);
         ktimer_end("sssp_do_round");
+        launch_finalize(R.persist_blocks, 0xF, FinSsspDo{(long long)g->edges});
       },
       sssp_round_cap(g), &rounds, &launches, 8);
   IGX_TRY(timer_stop(&info->seconds));

@@ -65,6 +65,7 @@ def lib():
         sig("orc_pr_fp64", ctypes.c_int, [G, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp])
         sig("orc_pr_check", ctypes.c_double, [G, fp])
         sig("orc_tc_serial", ctypes.c_ulonglong, [G])
+        sig("orc_tc_forward", ctypes.c_ulonglong, [G])
         sig("orc_tc_omp_vertex", ctypes.c_double, [G, ullp])
         sig("orc_tc_omp_edge", ctypes.c_double, [G, ip, ullp])
         sig("orc_num_threads", ctypes.c_int, [])
@@ -196,6 +197,11 @@ def pr_check(g, scores):
 
 def tc_serial(g):
     return int(lib().orc_tc_serial(ctypes.byref(_gs(g))))
+
+
+def tc_forward(g):
+    """Degree-ordered forward count (checker for large graphs)."""
+    return int(lib().orc_tc_forward(ctypes.byref(_gs(g))))
 
 
 def tc_omp_vertex(g):

@@ -61,7 +61,7 @@ def test_known_answers_small_inputs(fx):
     mis = O.mis_omp_push(g)[0]
     assert sorted(np.nonzero(mis == 1)[0].tolist()) == fx["mis"]
     assert np.array_equal(O.mis_greedy(g), mis)
-    assert O.tc_serial(g) == fx["tc"] and O.tc_omp_vertex(g)[0] == fx["tc"]
+    assert O.tc_serial(g) == fx["tc"] and O.tc_omp_vertex(g)[0] == fx["tc"] and O.tc_forward(g) == fx["tc"]
 
 
 @pytest.mark.skipif(not reference_present(), reason="reference tree not mounted")
@@ -135,7 +135,7 @@ def test_bfs_sssp_cc_tc_against_scipy(make):
     assert np.array_equal(cc, mins[lab])
     assert O.cc_verify(g, cc) == 0
     U = sps.triu(A, 1).tocsr()
-    assert O.tc_serial(g) == int((U @ U).multiply(U).sum())
+    assert O.tc_serial(g) == int((U @ U).multiply(U).sum()) == O.tc_forward(g)
 
 
 def test_cc_verify_detects_errors():
