@@ -336,6 +336,15 @@ struct FinWorklist {
   __device__ void operator()(Ctl* ctl, const RoundTotals&) const { finalize_worklist_round(ctl); }
 };
 
+// Visiting order for one-wave-per-vertex loops: k -> (k * mult) mod n with
+// mult coprime to n is a permutation of [0, n).  R-MAT's hubs sit at ids with
+// many zero bits, so a plain round-robin (v = wave + k * waves) hands dozens of
+// them to the same wave; the multiplicative order spreads them out.
+__device__ __forceinline__ int spread_vertex(int k, int n, unsigned mult)
+{
+  return (int)(((unsigned long long)(unsigned)k * mult) % (unsigned long long)n);
+}
+
 // MIS priority: the reference's integer hash of (v + 712313887), compared as
 // unsigned (mis_vertex_omp.h:127-132 and codes/omp MIS init).
 __device__ __forceinline__ uint32_t mis_priority(uint32_t v)

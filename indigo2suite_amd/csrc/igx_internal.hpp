@@ -99,6 +99,23 @@ inline void launch_finalize(int nparts, int stat_mask, Fin fin)
                      fin);
 }
 
+// multiplier for spread_vertex: odd, coprime to n
+inline unsigned spread_mult(int n)
+{
+  auto gcd = [](unsigned long long a, unsigned long long b) {
+    while (b != 0) {
+      const unsigned long long t = a % b;
+      a = b;
+      b = t;
+    }
+    return a;
+  };
+  unsigned m = 2654435761u % (unsigned)(n > 1 ? n : 2);
+  if (m == 0) m = 1;
+  while (gcd(m, (unsigned)n) != 1) m++;
+  return m;
+}
+
 inline int blocks_for(long long items, int per_block)
 {
   long long b = (items + per_block - 1) / per_block;

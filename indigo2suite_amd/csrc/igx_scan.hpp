@@ -27,7 +27,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int* lds, int* total)
 }
 
 // out[i] = sum of in[0..i) within the tile; tile_sum[b] = tile total
-__global__ void __launch_bounds__(kBlock) scan_tiles(const int* __restrict__ in, int* __restrict__ out, int count,
+static __global__ void __launch_bounds__(kBlock) scan_tiles(const int* __restrict__ in, int* __restrict__ out, int count,
                                                       int* __restrict__ tile_sum)
 {
   __shared__ int lds[kWavesPerBlock];
@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(kBlock) scan_tiles(const int* __restrict__ in,
 }
 
 // one workgroup: exclusive scan of the tile totals in place; total[0] = sum
-__global__ void __launch_bounds__(kBlock) scan_tile_sums(int* __restrict__ tile_sum, int tiles, int* __restrict__ total)
+static __global__ void __launch_bounds__(kBlock) scan_tile_sums(int* __restrict__ tile_sum, int tiles, int* __restrict__ total)
 {
   __shared__ int lds[kWavesPerBlock];
   int carry = 0;
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(kBlock) scan_tile_sums(int* __restrict__ tile_
   if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ void __launch_bounds__(kBlock) scan_add(int* __restrict__ out, int count, const int* __restrict__ tile_sum)
+static __global__ void __launch_bounds__(kBlock) scan_add(int* __restrict__ out, int count, const int* __restrict__ tile_sum)
 {
   const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (i < count) out[i] += tile_sum[i / kScanTile];

@@ -211,17 +211,20 @@ __global__ void __launch_bounds__(kBlock) mis_vertex_pass(int n, int* __restrict
 }
 
 // ------------------------------------------------------- resumable pull ----
-// Only the neighbours that outrank v matter: if one of them is in the set v is
-// out; if all of them are out v is in; a lower-ranked neighbour cannot be in
-// the set while v is undecided.  So an undecided vertex walks its outranking
-// neighbours in list order and stops at the first one not yet excluded: in
-// the set -> v is excluded; undecided -> v waits, and remembers where it
-// stopped (pos[v]: every outranking neighbour before it is excluded, which is
-// final), so a later round resumes there instead of rescanning the row.
-// Each lane walks at most kMisLane neighbours on its own per round; lanes with
-// more to walk are finished by the whole wave, 64 neighbours per step.  The
-// fixed point is the same greedy set; only the round in which a vertex is
-// decided can differ from the full-scan styles.
+// Only the neighbours that outrank v matter for v joining the set: v joins
+// once all of them are excluded.  A vertex that joins excludes all its
+// neighbours itself (a push: plain stores, every one of them is either
+// lower-ranked, and so cannot join while v is undecided or in, or already
+// excluded), so an undecided vertex only has to wait: it walks its
+// outranking neighbours in list order and stops at the first one that is not
+// excluded (in the set -> v is excluded; undecided -> v waits).  Where it
+// stopped is kept in pos[v] (every outranking neighbour before it is excluded,
+// which is final), so later rounds resume there instead of rescanning.
+// A lane walks at most kMisLane neighbours on its own per round and pushes
+// rows of at most kMisLane edges itself; longer walks and pushes are finished
+// by the whole wave, 64 neighbours per step.  The fixed point is the same
+// greedy set; only the round in which a vertex is decided can differ from the
+// full-scan styles.
 constexpr int kMisLane = 32;
 
 __global__ void __launch_bounds__(kBlock) mis_pos_init(const int* __restrict__ nindex, int* __restrict__ pos, int n)
@@ -291,12 +294,34 @@ __global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict_
         i = stop;
       }
     }
+    int rb = 0;
     if (undec) {
       if (result != kUndecided) {
         store_relaxed(&status[v], result);
+        rb = nindex[v];
       } else {
         pos[v] = i;
         pending = true;
+      }
+    }
+    // a vertex that joined excludes its neighbours
+    const bool joined = undec && result == kIncluded;
+    const bool short_row = joined && end - rb <= kMisLane;
+    if (short_row) {
+      for (int j = rb; j < end; j++) {
+        const int u = nlist[j];
+        if (u != v) store_relaxed(&status[u], kExcluded);
+      }
+    }
+    uint64_t big = ballot(joined && !short_row);
+    while (big != 0) {
+      const int L = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const int vL = base + L;
+      const int s0 = __shfl(rb, L, kWave), e0 = __shfl(end, L, kWave);
+      for (int j = s0 + lane; j < e0; j += kWave) {
+        const int u = nlist[j];
+        if (u != vL) store_relaxed(&status[u], kExcluded);
       }
     }
   }

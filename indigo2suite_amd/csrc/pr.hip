@@ -13,9 +13,10 @@
 //   pull_wave_kahan     PR_CUDA_Pull_..._Warp siblings (one wave64 per vertex)
 //   pull_block_kahan    PR_CUDA_Pull_..._Block siblings (one workgroup per vertex)
 //   pull_balanced_kahan (no reference counterpart) rows classed by degree:
-//                       thread per small row, wave per medium row, and
-//                       large rows split into chunks whose compensated partial
-//                       sums are merged in chunk order
+//                       small rows streamed in runs of consecutive rows and
+//                       summed in edge order by their lane, wave per medium
+//                       row, and large rows split into chunks whose
+//                       compensated partial sums are merged in chunk order
 // The contribution of the next iteration (score / degree) is written by the
 // same kernel that produces the score, so one pass per iteration remains.
 #include "igx_internal.hpp"
@@ -25,8 +26,9 @@ namespace igx {
 namespace {
 
 constexpr float kDamp = 0.85f;  // pr_cuda.h:43
-constexpr int kSmallRow = 32;
-constexpr int kMediumRow = 2048;
+constexpr int kSmallRow = 32;    // rows streamed in runs by pr_bal_main
+constexpr int kMediumRow = 2048; // rows with one wave each; longer ones by chunks
+constexpr int kSlice = 512;      // edges per streamed slice
 
 struct Pair {
   float s;  // running sum
@@ -206,22 +208,61 @@ __global__ void __launch_bounds__(kBlock) pr_bal_main(const int* __restrict__ ni
                                                        Ctl* ctl)
 {
   __shared__ double s_red[kWavesPerBlock];
+  __shared__ float s_val[kWavesPerBlock][kSlice];
   if (round_done(ctl)) return;
   double err = 0.0;
-  // small rows: one thread each
-  const int stride = gridDim.x * kBlock;
-  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
-    const int beg = nindex[v], end = nindex[v + 1];
-    if (end - beg <= kSmallRow) {
-      Pair acc{0.0f, 0.0f};
-      for (int i = beg; i < end; i++) two_sum_add(acc, contrib_cur[nlist[i]]);
-      err += pr_finish(v, acc, base, nindex, score, contrib_next);
-    }
-  }
-  // medium rows: one wave each
   const int waves = gridDim.x * kWavesPerBlock;
   const int gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   const int lane = lane_id();
+  // small rows: 64-row tiles; consecutive small rows form runs whose edges
+  // are contiguous, streamed kSlice at a time (independent loads, values
+  // parked in LDS); each lane then adds its own row's values in edge order
+  float* vbuf = s_val[threadIdx.x / kWave];
+  for (int v0 = gw * kWave; v0 < n; v0 += waves * kWave) {  // uniform per wave
+    const int v = v0 + lane;
+    int rb = 0, re = 0;
+    if (v < n) {
+      rb = nindex[v];
+      re = nindex[v + 1];
+    }
+    const bool small = (v < n) && (re - rb <= kSmallRow);
+    Pair acc{0.0f, 0.0f};
+    uint64_t todo = ballot(small);
+    while (todo != 0) {  // one run of consecutive small rows
+      const int first = __ffsll((unsigned long long)todo) - 1;
+      const uint64_t from = todo >> first;
+      const int cnt = (~from == 0) ? (kWave - first) : (__ffsll((unsigned long long)~from) - 1);
+      const int last = first + cnt - 1;
+      todo &= (cnt + first >= kWave) ? ((1ull << first) - 1) : ~(((1ull << cnt) - 1) << first);
+      const int r0 = __shfl(rb, first, kWave), r1 = __shfl(re, last, kWave);
+      const bool mine = lane >= first && lane <= last;
+      for (int c0 = r0; c0 < r1; c0 += kSlice) {
+        const int c1 = (r1 - c0 < kSlice) ? r1 : c0 + kSlice;
+        int u[kSlice / kWave];
+        #pragma unroll
+        for (int k = 0; k < kSlice / kWave; k++) {
+          const int e = c0 + k * kWave + lane;
+          u[k] = (e < c1) ? nlist[e] : -1;
+        }
+        float x[kSlice / kWave];
+        #pragma unroll
+        for (int k = 0; k < kSlice / kWave; k++) x[k] = (u[k] >= 0) ? contrib_cur[u[k]] : 0.0f;
+        __builtin_amdgcn_wave_barrier();
+        #pragma unroll
+        for (int k = 0; k < kSlice / kWave; k++) vbuf[k * kWave + lane] = x[k];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (mine) {
+          const int lo = (rb > c0) ? rb : c0;
+          const int hi = (re < c1) ? re : c1;
+          for (int i = lo; i < hi; i++) two_sum_add(acc, vbuf[i - c0]);
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    if (small) err += pr_finish(v, acc, base, nindex, score, contrib_next);
+  }
+  // medium rows: one wave each
   const int nmed = counts[0];
   for (int k = gw; k < nmed; k += waves) {
     const int v = medium[k];

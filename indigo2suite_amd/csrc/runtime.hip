@@ -9,6 +9,7 @@
 
 #include "igx_internal.hpp"
 #include "igx_frontier.hpp"
+#include "igx_scan.hpp"
 
 namespace igx {
 
@@ -134,120 +135,108 @@ int device_sp(const igx_graph* g, const int** d_sp)
 }
 
 // ---------------------------------------------------- long-row chunk list
-// Block b owns the contiguous rows [b * span, (b + 1) * span), span a
-// multiple of the block size, and scans them twice: big_rows_count stores the
-// number of chunks of its long rows, big_rows_write sums the counts of the
-// blocks before it and writes its chunks there, in row order.
-__device__ __forceinline__ void big_rows_span(int n, int* v0, int* v1)
+// No atomics, balanced however the long rows are placed: big_rows_tiles
+// counts the chunks of each 64-row tile (one wave per tile), a device-wide
+// exclusive scan turns the counts into offsets, and big_rows_fill gives every
+// wave an equal slice of the output: it finds the tile holding its first
+// chunk by binary search over the offsets and writes its slice tile by tile
+// (R-MAT's first tile alone holds thousands of hub chunks; a wave per tile
+// would write them alone).
+__device__ __forceinline__ int long_row_chunks(const int* __restrict__ nindex, int n, int v)
 {
-  const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
-  const int span = (per + kBlock - 1) / kBlock * kBlock;
-  const long long lo = (long long)blockIdx.x * span;
-  *v0 = (lo < n) ? (int)lo : n;
-  *v1 = (n - *v0 < span) ? n : *v0 + span;
+  if (v >= n) return 0;
+  const int d = nindex[v + 1] - nindex[v];
+  return (d > kChunk) ? chunks_of(d) : 0;
 }
 
-__global__ void __launch_bounds__(kBlock) big_rows_count(const int* __restrict__ nindex, int n,
-                                                          int* __restrict__ block_counts)
+__global__ void __launch_bounds__(kBlock) big_rows_tiles(const int* __restrict__ nindex, int n,
+                                                          int* __restrict__ tile_count)
 {
-  __shared__ int s_wave[kWavesPerBlock];
-  int v0, v1;
-  big_rows_span(n, &v0, &v1);
-  int mine = 0;
-  for (int v = v0 + (int)threadIdx.x; v < v1; v += kBlock) {
-    const int d = nindex[v + 1] - nindex[v];
-    if (d > kChunk) mine += chunks_of(d);
+  const int ntiles = (n + kWave - 1) / kWave;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  for (int t = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; t < ntiles; t += nwaves) {
+    const int c = wave_sum(long_row_chunks(nindex, n, t * kWave + lane_id()));
+    if (lane_id() == 0) tile_count[t] = c;
   }
-  mine = wave_sum(mine);
-  if (lane_id() == 0) s_wave[threadIdx.x / kWave] = mine;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int k = 0; k < kWavesPerBlock; k++) t += s_wave[k];
-    block_counts[blockIdx.x] = t;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) tile_count[ntiles] = 0;  // so the scan's last offset is the total
 }
 
-__global__ void __launch_bounds__(kBlock) big_rows_write(const int* __restrict__ nindex, int n,
-                                                          const int* __restrict__ block_counts,
-                                                          Entry* __restrict__ bigq, int* __restrict__ count)
+__global__ void __launch_bounds__(kBlock) big_rows_fill(const int* __restrict__ nindex, int n,
+                                                         const int* __restrict__ tile_off, Entry* __restrict__ bigq)
 {
-  __shared__ int s_wave[kWavesPerBlock];
-  __shared__ int s_wave2[kWavesPerBlock];
+  __shared__ int s_pre[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
+  __shared__ int s_end[kWavesPerBlock][kWave];
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
-  // offset of this block (and, in block 0, the total)
-  int before = 0, all = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
-    const int c = block_counts[b];
-    all += c;
-    if (b < (int)blockIdx.x) before += c;
-  }
-  before = wave_sum(before);
-  all = wave_sum(all);
-  if (lane == 0) {
-    s_wave[w] = before;
-    s_wave2[w] = all;
-  }
-  __syncthreads();
-  int run = 0, tot = 0;
-  for (int k = 0; k < kWavesPerBlock; k++) {
-    run += s_wave[k];
-    tot += s_wave2[k];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *count = tot;
-  __shared__ int s_pos[kBlock];
-  __shared__ int s_beg[kBlock];
-  __shared__ int s_end[kBlock];
-  int v0, v1;
-  big_rows_span(n, &v0, &v1);
-  for (int b0 = v0; b0 < v1; b0 += kBlock) {  // uniform per block
-    const int v = b0 + (int)threadIdx.x;
-    int c = 0, beg = 0, end = 0;
-    if (v < v1) {
-      beg = nindex[v];
-      end = nindex[v + 1];
-      if (end - beg > kChunk) c = chunks_of(end - beg);
+  int* pre = s_pre[w];
+  int* rbeg = s_beg[w];
+  int* rend = s_end[w];
+  const int ntiles = (n + kWave - 1) / kWave;
+  const int nwaves = gridDim.x * kWavesPerBlock;
+  const int gw = blockIdx.x * kWavesPerBlock + w;
+  const int total = tile_off[ntiles];
+  const int per = (total + nwaves - 1) / nwaves;
+  const long long j0l = (long long)gw * per;
+  if (per == 0 || j0l >= total) return;  // uniform per wave
+  const int j0 = (int)j0l;
+  const int j1 = (total - j0 < per) ? total : j0 + per;
+  for (int j = j0; j < j1;) {  // uniform per wave
+    // the tile holding chunk j, the last t with tile_off[t] <= j, by a 64-way
+    // search: each step the lanes probe 64 evenly spaced tiles of the current
+    // range (three dependent loads for 2^18 tiles instead of eighteen)
+    int lo = 0, span = ntiles;
+    while (span > 1) {
+      const int step = (span + kWave - 1) / kWave;
+      const int idx = lo + lane * step;
+      const bool le = (lane * step < span) && idx < ntiles && tile_off[idx] <= j;
+      const uint64_t m = ballot(le);  // lane 0 always holds (tile_off[lo] <= j)
+      lo += (63 - __clzll((long long)m)) * step;
+      span = step;
     }
+    const int t = lo;
+    const int t0 = tile_off[t], t1 = tile_off[t + 1];
+    const int v = t * kWave + lane;
+    const int c = long_row_chunks(nindex, n, v);
     const int incl = wave_incl_scan(c);
-    __syncthreads();
-    if (lane == kWave - 1) s_wave[w] = incl;
-    __syncthreads();
-    int woff = 0, btot = 0;
-    for (int k = 0; k < kWavesPerBlock; k++) {
-      if (k < w) woff += s_wave[k];
-      btot += s_wave[k];
-    }
-    s_pos[threadIdx.x] = woff + incl - c;
-    s_beg[threadIdx.x] = beg;
-    s_end[threadIdx.x] = end;
-    __syncthreads();
-    // the block writes the tile's chunks together: chunk j belongs to the
-    // last row k with s_pos[k] <= j (rows without chunks share the next offset)
-    for (int j = threadIdx.x; j < btot; j += kBlock) {
-      int lo = 0, hi = kBlock;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_pos[mid] <= j) lo = mid;
-        else hi = mid;
+    __builtin_amdgcn_wave_barrier();
+    pre[lane] = incl - c;
+    rbeg[lane] = (c > 0) ? nindex[v] : 0;
+    rend[lane] = (c > 0) ? nindex[v + 1] : 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const int b = ((j1 < t1) ? j1 : t1) - t0;  // this wave's part of the tile: [j - t0, b)
+    for (int q = j - t0 + lane; q < b; q += kWave) {
+      int l = 0, h = kWave;  // last row k with pre[k] <= q
+      while (h - l > 1) {
+        const int mid = (l + h) >> 1;
+        if (pre[mid] <= q) l = mid;
+        else h = mid;
       }
-      const int b = s_beg[lo] + (j - s_pos[lo]) * kChunk;
-      const int e = s_end[lo];
-      bigq[run + j] = Entry{b0 + lo, b, (b + kChunk < e) ? b + kChunk : e, 0};
+      const int rb = rbeg[l], re = rend[l];
+      const int beg = rb + (q - pre[l]) * kChunk;
+      bigq[t0 + q] = Entry{t * kWave + l, beg, (beg + kChunk < re) ? beg + kChunk : re, 0};
     }
-    run += btot;
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
+    j = t0 + b;
   }
 }
 
 int big_rows(const igx_graph* g, void* bigq, int* d_count)
 {
   const int grid = g_rt.persist_blocks;
+  const int ntiles = (g->nodes + kWave - 1) / kWave;
+  const int cnt = ntiles + 1;
+  const int tsum = scan_tiles_for(cnt);
   void* pb = nullptr;
-  IGX_TRY(ws_get(WS_BLOCKS, sizeof(int) * (size_t)grid, &pb));
-  hipLaunchKernelGGL(big_rows_count, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes, (int*)pb);
-  hipLaunchKernelGGL(big_rows_write, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes,
-                     (const int*)pb, (Entry*)bigq, d_count);
+  IGX_TRY(ws_get(WS_BLOCKS, sizeof(int) * (2 * (size_t)cnt + tsum), &pb));
+  int* tile_count = (int*)pb;
+  int* tile_off = tile_count + cnt;
+  int* tile_sum = tile_off + cnt;
+  hipLaunchKernelGGL(big_rows_tiles, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes, tile_count);
+  exclusive_scan(tile_count, tile_off, cnt, tile_sum, d_count, g_rt.stream);
+  hipLaunchKernelGGL(big_rows_fill, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes,
+                     (const int*)tile_off, (Entry*)bigq);
   return IGX_OK;
 }
 

@@ -13,12 +13,13 @@
 //                    bound reads one past the list)
 //   oriented_lds     (no reference counterpart) degree-ordered orientation:
 //                    each edge points from lower to higher (degree, id) rank,
-//                    so hub lists become short; one wave per vertex stages its
-//                    out-list in LDS and every lane probes it for the
-//                    out-neighbours of one out-neighbour
+//                    so hub lists become short; one wave per vertex keeps its
+//                    out-list as a hash set in LDS and probes the out-lists of
+//                    its out-neighbours into it, 64 elements per step
 // Counts are reduced per wave and added to one uint64 with a single atomic
 // per wave (the reference's GlobalAdd/BlockAdd/Reduction axis).
 #include "igx_internal.hpp"
+#include "igx_frontier.hpp"
 #include "igx_scan.hpp"
 
 namespace igx {
@@ -33,12 +34,13 @@ __global__ void tc_zero(unsigned long long* __restrict__ c, int* __restrict__ fl
 }
 
 __global__ void __launch_bounds__(kBlock) tc_check_sorted(const int* __restrict__ nindex, const int* __restrict__ nlist,
-                                                           int n, int* __restrict__ bad)
+                                                           int n, unsigned mult, int* __restrict__ bad)
 {
   const int waves = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
   bool b = false;
-  for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+  for (int k = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n; k += waves) {
+    const int v = spread_vertex(k, n, mult);
     for (int i = nindex[v] + 1 + lane; i < nindex[v + 1]; i += kWave) {
       if (nlist[i - 1] >= nlist[i]) b = true;
     }
@@ -154,12 +156,13 @@ __device__ __forceinline__ bool rank_above(int du, int u, int dv, int v)
 }
 
 __global__ void __launch_bounds__(kBlock) tc_orient_count(const int* __restrict__ nindex,
-                                                           const int* __restrict__ nlist, int n,
+                                                           const int* __restrict__ nlist, int n, unsigned mult,
                                                            int* __restrict__ ocount)
 {
   const int waves = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
-  for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+  for (int k = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n; k += waves) {
+    const int v = spread_vertex(k, n, mult);
     const int beg = nindex[v], end = nindex[v + 1];
     const int dv = end - beg;
     int c = 0;
@@ -173,12 +176,13 @@ __global__ void __launch_bounds__(kBlock) tc_orient_count(const int* __restrict_
 }
 
 __global__ void __launch_bounds__(kBlock) tc_orient_fill(const int* __restrict__ nindex,
-                                                          const int* __restrict__ nlist, int n,
+                                                          const int* __restrict__ nlist, int n, unsigned mult,
                                                           const int* __restrict__ onidx, int* __restrict__ olist)
 {
   const int waves = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
-  for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+  for (int k = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n; k += waves) {
+    const int v = spread_vertex(k, n, mult);
     const int beg = nindex[v], end = nindex[v + 1];
     const int dv = end - beg;
     int out = onidx[v];
@@ -197,49 +201,110 @@ __global__ void __launch_bounds__(kBlock) tc_orient_fill(const int* __restrict__
   }
 }
 
-constexpr int kLdsList = 1024;  // out-list entries staged per wave
+// Count per vertex v: every oriented edge (v, u) adds |N+(v) ∩ N+(u)|.  One
+// wave per v builds an open-addressing hash set of N+(v) in LDS (the table is
+// the power of two >= 2 |N+(v)|, multiplicative hashing, linear probing), then
+// walks the concatenated out-lists of v's out-neighbours 64 elements per step
+// (wave merge path, so short and long lists share the lanes evenly) and probes
+// each element once.  Out-lists longer than kHashMax / 2 fall back to a
+// per-lane binary search of N+(v) in memory.
+constexpr int kHashMax = 2048;  // hash slots per wave
+
+__device__ __forceinline__ unsigned tc_hash(int x, int bits)
+{
+  return ((unsigned)x * 0x9E3779B1u) >> (32 - bits);
+}
 
 __global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restrict__ onidx,
-                                                             const int* __restrict__ olist, int n,
+                                                             const int* __restrict__ olist, int n, unsigned mult,
                                                              unsigned long long* __restrict__ count)
 {
-  __shared__ int s_list[kWavesPerBlock][kLdsList];
+  __shared__ int s_tab[kWavesPerBlock][kHashMax];
+  __shared__ int s_excl[kWavesPerBlock][kWave];
+  __shared__ Entry s_ent[kWavesPerBlock][kWave];
   const int w = threadIdx.x / kWave;
   const int waves = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
-  int* lds = s_list[w];
+  int* tab = s_tab[w];
+  int* excl = s_excl[w];
+  Entry* ents = s_ent[w];
   unsigned long long c = 0;
-  for (int v = blockIdx.x * kWavesPerBlock + w; v < n; v += waves) {
+  for (int k = blockIdx.x * kWavesPerBlock + w; k < n; k += waves) {  // uniform per wave
+    const int v = spread_vertex(k, n, mult);
     const int beg = onidx[v], end = onidx[v + 1];
     const int len = end - beg;
     if (len < 2) continue;
-    const bool staged = len <= kLdsList;
-    if (staged) {
-      for (int i = lane; i < len; i += kWave) lds[i] = olist[beg + i];
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    for (int j = beg + lane; j < end; j += kWave) {
-      const int u = olist[j];
-      for (int k = onidx[u]; k < onidx[u + 1]; k++) {
-        const int x = olist[k];
-        int lo = 0, hi = len;
-        bool hit = false;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          const int y = staged ? lds[mid] : olist[beg + mid];
-          if (y == x) {
-            hit = true;
-            break;
+    if (2 * len > kHashMax) {
+      for (int j = beg + lane; j < end; j += kWave) {
+        const int u = olist[j];
+        for (int k = onidx[u]; k < onidx[u + 1]; k++) {
+          const int x = olist[k];
+          int lo = beg, hi = end;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const int y = olist[mid];
+            if (y < x) lo = mid + 1;
+            else hi = mid;
           }
-          if (y < x) lo = mid + 1;
-          else hi = mid;
+          c += (lo < end && olist[lo] == x) ? 1u : 0u;
         }
-        c += hit ? 1u : 0u;
       }
+      continue;
+    }
+    int bits = 1;
+    while ((1 << bits) < 2 * len) bits++;
+    const int size = 1 << bits;
+    for (int i = lane; i < size; i += kWave) tab[i] = -1;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int i = lane; i < len; i += kWave) {
+      const int x = olist[beg + i];
+      unsigned h = tc_hash(x, bits);
+      while (atomicCAS(&tab[h], -1, x) != -1) h = (h + 1) & (size - 1);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int gb = beg; gb < end; gb += kWave) {
+      const int j = gb + lane;
+      Entry ent = Entry{0, 0, 0, 0};
+      if (j < end) {
+        const int u = olist[j];
+        ent = Entry{u, onidx[u], onidx[u + 1], 0};
+      }
+      const int el = ent.end - ent.beg;
+      const int incl = wave_incl_scan(el);
+      const int total = wave_bcast(incl, kWave - 1);
+      if (total == 0) continue;  // uniform
+      __builtin_amdgcn_wave_barrier();
+      excl[lane] = incl - el;
+      ents[lane] = ent;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      for (int tb = 0; tb < total; tb += kWave) {
+        const int t = tb + lane;
+        if (t < total) {
+          int lo = 0, hi = kWave;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (excl[mid] <= t) lo = mid;
+            else hi = mid;
+          }
+          const int x = olist[ents[lo].beg + (t - excl[lo])];
+          unsigned h = tc_hash(x, bits);
+          while (true) {
+            const int y = tab[h];
+            if (y == x) {
+              c += 1u;
+              break;
+            }
+            if (y < 0) break;
+            h = (h + 1) & (size - 1);
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_wave_barrier();
   }
   wave_add_count(count, c);
 }
@@ -253,7 +318,7 @@ int check_sorted(const igx_graph* g)
   int* bad = (int*)p;
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, (unsigned long long*)nullptr, bad);
   hipLaunchKernelGGL(tc_check_sorted, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes,
-                     bad);
+                     spread_mult(g->nodes), bad);
   int h = 0;
   IGX_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, R.stream));
   IGX_HIP(hipStreamSynchronize(R.stream));
@@ -314,16 +379,19 @@ int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info
   int* tiles = (int*)ptile;
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
-  hipLaunchKernelGGL(tc_orient_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n,
+  const unsigned mult = spread_mult(n);
+  hipLaunchKernelGGL(tc_orient_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, mult,
                      ocount);
   // scan n+1 entries so that onidx[n] is the total; ocount[n] is zeroed first
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, (unsigned long long*)nullptr, ocount + n);
   exclusive_scan(ocount, onidx, n + 1, tiles, tiles + scan_tiles_for(n + 1), R.stream);
-  hipLaunchKernelGGL(tc_orient_fill, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, onidx,
-                     (int*)plist);
+  hipLaunchKernelGGL(tc_orient_fill, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, mult,
+                     onidx, (int*)plist);
   ktimer_begin("tc_oriented_count");
+  // identity order here: neighbouring vertices share out-lists in cache, which
+  // outweighs the imbalance the spread order removes
   hipLaunchKernelGGL(tc_oriented_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, onidx, (const int*)plist, n,
-                     count);
+                     1u, count);
   ktimer_end("tc_oriented_count");
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
