@@ -140,6 +140,21 @@ def pr_tc_section(args, D, G):
     return out
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "sssp_do_round_rmat22_traffic.md")
+
+
+def measured_traffic(scale):
+    """HBM bytes per launch of the roofline kernel at scale 22, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated);
+    None for other scales or when the summary is absent."""
+    if scale != 22 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(scale):
     import oracle as O
     from indigo2suite_amd import graph as G
@@ -164,6 +179,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip PageRank and TC on R-MAT 20 and the grid")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -269,10 +287,18 @@ def main():
     bytes_total = 4 * work_e + 8 * work_r + 12 * work_v + 4 * launches
     achieved = bytes_total / total_s * 1e-9 if total_s > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.scale),
                 "bytes_per_launch": int(bytes_total / launches), "avg_launch_us": round(total_s / launches * 1e6, 2),
                 "launches": hook_n.value, "edges_scanned_per_run": int(work_e / args.steps),
                 "edges_relaxed_per_run": int(work_r / args.steps), "rows_visited_per_run": int(work_v / args.steps)}
+
+    if args.copy_probe > 0:
+        nb = args.copy_probe << 20
+        a, b = D.DeviceBuffer(nb), D.DeviceBuffer(nb)
+        D.check(lib.igx_memcpy_d2d(b.ptr, a.ptr, nb), "igx_memcpy_d2d")
+        a.free()
+        b.free()
+        log(f"copy probe: {nb} bytes device to device")
 
     cpu = None
     if not args.no_cpu:

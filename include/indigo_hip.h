@@ -49,6 +49,8 @@ int igx_free(void* d_ptr);
 int igx_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes);
 int igx_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes);
 int igx_memset(void* d_ptr, int value, size_t bytes);
+/* device-to-device copy on the library stream (waits for completion) */
+int igx_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes);
 
 /* Device copy of a host CSR: same struct, device pointers (eweight only if
  * the host graph has weights).  Replaces the harness upload

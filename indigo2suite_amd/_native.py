@@ -124,6 +124,7 @@ def hip_lib():
         _sig(lib, "igx_memcpy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t])
         _sig(lib, "igx_memcpy_d2h", ctypes.c_int, [vp, vp, ctypes.c_size_t])
         _sig(lib, "igx_memset", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_size_t])
+        _sig(lib, "igx_memcpy_d2d", ctypes.c_int, [vp, vp, ctypes.c_size_t])
         _sig(lib, "igx_graph_upload", ctypes.c_int, [G, G])
         _sig(lib, "igx_graph_free_device", ctypes.c_int, [G])
         _sig(lib, "igx_build_sp", ctypes.c_int, [G, ctypes.POINTER(ip)])

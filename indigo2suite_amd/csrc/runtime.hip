@@ -445,6 +445,15 @@ int igx_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes)
   return IGX_OK;
 }
 
+int igx_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes)
+{
+  IGX_TRY(ensure_init());
+  if (bytes == 0) return IGX_OK;
+  IGX_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, g_rt.stream));
+  IGX_HIP(hipStreamSynchronize(g_rt.stream));
+  return IGX_OK;
+}
+
 int igx_memset(void* d_ptr, int value, size_t bytes)
 {
   IGX_TRY(ensure_init());

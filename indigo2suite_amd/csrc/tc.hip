@@ -204,11 +204,14 @@ __global__ void __launch_bounds__(kBlock) tc_orient_fill(const int* __restrict__
 // Count per vertex v: every oriented edge (v, u) adds |N+(v) ∩ N+(u)|.  One
 // wave per v builds an open-addressing hash set of N+(v) in LDS (the table is
 // the power of two >= 2 |N+(v)|, multiplicative hashing, linear probing), then
-// walks the concatenated out-lists of v's out-neighbours 64 elements per step
-// (wave merge path, so short and long lists share the lanes evenly) and probes
-// each element once.  Out-lists longer than kHashMax / 2 fall back to a
-// per-lane binary search of N+(v) in memory.
-constexpr int kHashMax = 2048;  // hash slots per wave
+// walks the concatenated out-lists of v's out-neighbours (wave merge path, so
+// short and long lists share the lanes evenly), kIlp elements per lane per
+// step with their loads issued together, and probes each element once.
+// Out-lists longer than kHashMax / 2 fall back to a per-lane binary search of
+// N+(v) in memory.  Launched with several workgroups per CU slot so that more
+// waves hide the latency of the probe chain.
+constexpr int kHashMax = 1024;  // hash slots per wave
+constexpr int kIlp = 4;
 
 __device__ __forceinline__ unsigned tc_hash(int x, int bits)
 {
@@ -221,13 +224,13 @@ __global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restric
 {
   __shared__ int s_tab[kWavesPerBlock][kHashMax];
   __shared__ int s_excl[kWavesPerBlock][kWave];
-  __shared__ Entry s_ent[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
   const int w = threadIdx.x / kWave;
   const int waves = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
   int* tab = s_tab[w];
   int* excl = s_excl[w];
-  Entry* ents = s_ent[w];
+  int* ebeg = s_beg[w];
   unsigned long long c = 0;
   for (int k = blockIdx.x * kWavesPerBlock + w; k < n; k += waves) {  // uniform per wave
     const int v = spread_vertex(k, n, mult);
@@ -237,13 +240,12 @@ __global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restric
     if (2 * len > kHashMax) {
       for (int j = beg + lane; j < end; j += kWave) {
         const int u = olist[j];
-        for (int k = onidx[u]; k < onidx[u + 1]; k++) {
-          const int x = olist[k];
+        for (int q = onidx[u]; q < onidx[u + 1]; q++) {
+          const int x = olist[q];
           int lo = beg, hi = end;
           while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            const int y = olist[mid];
-            if (y < x) lo = mid + 1;
+            if (olist[mid] < x) lo = mid + 1;
             else hi = mid;
           }
           c += (lo < end && olist[lo] == x) ? 1u : 0u;
@@ -266,34 +268,46 @@ __global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restric
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     for (int gb = beg; gb < end; gb += kWave) {
       const int j = gb + lane;
-      Entry ent = Entry{0, 0, 0, 0};
+      int ob = 0, oe = 0;
       if (j < end) {
         const int u = olist[j];
-        ent = Entry{u, onidx[u], onidx[u + 1], 0};
+        ob = onidx[u];
+        oe = onidx[u + 1];
       }
-      const int el = ent.end - ent.beg;
+      const int el = oe - ob;
       const int incl = wave_incl_scan(el);
       const int total = wave_bcast(incl, kWave - 1);
       if (total == 0) continue;  // uniform
       __builtin_amdgcn_wave_barrier();
       excl[lane] = incl - el;
-      ents[lane] = ent;
+      ebeg[lane] = ob;
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      for (int tb = 0; tb < total; tb += kWave) {
-        const int t = tb + lane;
-        if (t < total) {
-          int lo = 0, hi = kWave;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (excl[mid] <= t) lo = mid;
-            else hi = mid;
+      for (int tb = 0; tb < total; tb += kIlp * kWave) {
+        int x[kIlp];
+        #pragma unroll
+        for (int q = 0; q < kIlp; q++) {
+          const int t = tb + q * kWave + lane;
+          int lo = 0;
+          if (t < total) {
+            int hi = kWave;  // last k with excl[k] <= t
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (excl[mid] <= t) lo = mid;
+              else hi = mid;
+            }
           }
-          const int x = olist[ents[lo].beg + (t - excl[lo])];
-          unsigned h = tc_hash(x, bits);
+          x[q] = (t < total) ? ebeg[lo] + (t - excl[lo]) : -1;
+        }
+        #pragma unroll
+        for (int q = 0; q < kIlp; q++) x[q] = (x[q] >= 0) ? olist[x[q]] : -1;
+        #pragma unroll
+        for (int q = 0; q < kIlp; q++) {
+          if (x[q] < 0) continue;
+          unsigned h = tc_hash(x[q], bits);
           while (true) {
             const int y = tab[h];
-            if (y == x) {
+            if (y == x[q]) {
               c += 1u;
               break;
             }
@@ -390,8 +404,8 @@ int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info
   ktimer_begin("tc_oriented_count");
   // identity order here: neighbouring vertices share out-lists in cache, which
   // outweighs the imbalance the spread order removes
-  hipLaunchKernelGGL(tc_oriented_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, onidx, (const int*)plist, n,
-                     1u, count);
+  hipLaunchKernelGGL(tc_oriented_count, dim3(4 * R.persist_blocks), dim3(kBlock), 0, R.stream, onidx,
+                     (const int*)plist, n, 1u, count);
   ktimer_end("tc_oriented_count");
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
