@@ -1,9 +1,9 @@
 """Exhaustive style autotuner and the synthetic graph families it trains on.
 
 `sweep` times every registered style of an algorithm on one device graph
-(median of `repeats` runs, each run's own hipEvent time); a style that is
-slower than `cap_factor` x the best seen so far after its first run is not
-repeated.  `collect` builds the selector's training data: seeded graphs from
+(median of `repeats` runs, each run's own hipEvent time), the library's
+default style first; a style whose first run is slower than `cap_factor` x
+the best seen so far, or than `slow_seconds`, is not repeated.  `collect` builds the selector's training data: seeded graphs from
 four families (Erdos-Renyi; R-MAT with a drawn in [0.45, 0.65] and b:c:d kept
 at 0.19:0.19:0.05; 2-D grids with long-range edges; Chung-Lu power laws with
 exponent in [2.1, 3.0]), their 16 features, and the sweep of every style.
@@ -53,14 +53,35 @@ def build_from_spec(spec: dict) -> Graph:
     return g.with_weights(1, 255, seed=spec["seed"])
 
 
+# Triangle-counting styles without orientation do O(d(v)^2) work per vertex
+# in one thread or wave: on a graph whose largest degree is in the tens of
+# thousands a single run takes minutes.  They are not run there; their time
+# is recorded as +inf ("too slow to measure"), which the selector learns as
+# the largest slowdown.
+UNORIENTED_TC = ("v_merge_thread", "v_merge_wave", "e_bsearch_thread")
+TC_MAX_DEGREE = 8192
+
+
+def too_slow(algo: str, style: str, max_degree: int) -> bool:
+    return algo == "tc" and style in UNORIENTED_TC and max_degree > TC_MAX_DEGREE
+
+
 def sweep(dg: device.DeviceGraph, algo: str, src: int = 0, repeats: int = 3, cap_factor: float = 20.0,
-          styles: Optional[List[str]] = None) -> Dict[str, float]:
-    styles = styles or device.styles(algo)
+          styles: Optional[List[str]] = None, slow_seconds: float = 1.0, max_degree: int = 0) -> Dict[str, float]:
+    styles = list(styles or device.styles(algo))
+    # the library's default style first: its time caps the repeats of the slow ones
+    default = device.styles(algo)[device.default_entry_style(algo, 0)]
+    if default in styles:
+        styles.remove(default)
+        styles.insert(0, default)
     times: Dict[str, float] = {}
     best = math.inf
     for st in styles:
+        if too_slow(algo, st, max_degree):
+            times[st] = math.inf
+            continue
         first = device.run(algo, dg, st, src).seconds
-        if first > cap_factor * best:
+        if first > cap_factor * best or first > slow_seconds:
             times[st] = first
             continue
         ts = [first] + [device.run(algo, dg, st, src).seconds for _ in range(repeats - 1)]
@@ -70,31 +91,44 @@ def sweep(dg: device.DeviceGraph, algo: str, src: int = 0, repeats: int = 3, cap
 
 
 def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_base: int = 100000,
-            log_n_range=(12, 21), deg_range=(4, 32), repeats: int = 3, log=print) -> dict:
-    """Features and per-style times for `count` seeded graphs."""
+            log_n_range=(12, 21), deg_range=(4, 32), repeats: int = 3, log=print, checkpoint=None) -> dict:
+    """Features and per-style times for `count` seeded graphs.  `log` gets a
+    line per graph; checkpoint(partial_dataset) is called after each graph."""
     device.init()
     feats = []
     times = {a: [] for a in algos}
     specs = []
     t0 = time.time()
-    for i in range(count):
-        spec = graph_spec(i, seed_base, log_n_range, deg_range)
-        g = build_from_spec(spec)
-        src = g.max_degree_vertex()
-        feats.append(g.features(seed=spec["seed"]))
-        with device.DeviceGraph(g) as dg:
+
+    def dataset():
+        return {"features": np.array(feats), "times": {a: np.array(v) for a, v in times.items()},
+                "styles": {a: device.styles(a) for a in algos}, "specs": specs}
+
+    try:
+        for i in range(count):
+            spec = graph_spec(i, seed_base, log_n_range, deg_range)
+            g = build_from_spec(spec)
+            src = g.max_degree_vertex()
+            feat = g.features(seed=spec["seed"])
+            row = {}
+            dmax = int(np.diff(g.nindex).max()) if g.nodes > 0 else 0
+            with device.DeviceGraph(g) as dg:
+                for a in algos:
+                    device.set_pr_params(1e-6, 100) if a == "pr" else device.set_pr_params(1e-4, 100)
+                    sw = sweep(dg, a, src, repeats, max_degree=dmax)
+                    row[a] = [sw[s] for s in device.styles(a)]
+            feats.append(feat)
             for a in algos:
-                if a == "pr":
-                    device.set_pr_params(1e-6, 100)
-                sw = sweep(dg, a, src, repeats)
-                times[a].append([sw[s] for s in device.styles(a)])
-        spec.update({"n": g.nodes, "m": g.edges})
-        specs.append(spec)
-        if log and (i + 1) % max(1, count // 20) == 0:
-            log(f"collected {i + 1}/{count} graphs in {time.time() - t0:.0f}s")
-    device.set_pr_params(1e-4, 100)
-    return {"features": np.array(feats), "times": {a: np.array(v) for a, v in times.items()},
-            "styles": {a: device.styles(a) for a in algos}, "specs": specs}
+                times[a].append(row[a])
+            spec.update({"n": g.nodes, "m": g.edges})
+            specs.append(spec)
+            if log:
+                log(f"graph {i + 1}/{count} {spec['family']} n={g.nodes} m={g.edges} done at {time.time() - t0:.0f}s")
+            if checkpoint:
+                checkpoint(dataset())
+    finally:
+        device.set_pr_params(1e-4, 100)
+    return dataset()
 
 
 def save_dataset(path: str, data: dict) -> None:

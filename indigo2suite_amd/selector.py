@@ -12,6 +12,7 @@ training set's mean and standard deviation, which are saved with the weights.
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import List, Optional
 
 import numpy as np
@@ -114,12 +115,15 @@ def unpack(p: np.ndarray, n_in: int, n_out: int):
 
 
 def targets_from_times(times: np.ndarray):
-    """(y, mask): y = log(t / t_best) clipped at log(kClip); NaN/inf times masked."""
+    """(y, mask): y = log(t / t_best) clipped at log(kClip).  +inf (a style
+    too slow to run on that graph) gets the clipped maximum; NaN is masked."""
     t = np.asarray(times, np.float64)
     ok = np.isfinite(t) & (t > 0)
+    slow = np.isposinf(t)
     best = np.where(ok, t, np.inf).min(axis=1, keepdims=True)
     y = np.where(ok, np.log(np.minimum(t / best, kClip)), 0.0)
-    return y.astype(np.float32), ok.astype(np.float32)
+    y = np.where(slow, np.log(kClip), y)
+    return y.astype(np.float32), (ok | slow).astype(np.float32)
 
 
 class Selector:
@@ -148,7 +152,17 @@ class Selector:
         return self.net.forward(self._norm(np.atleast_2d(features)))
 
     def pick(self, features: np.ndarray) -> List[int]:
-        return list(np.argmin(self.predict_scores(features), axis=1))
+        """Style index with the lowest predicted slowdown, among the styles
+        the autotuner would run on that graph (feature 4 is log2(max degree + 1))."""
+        from .autotune import too_slow
+        f = np.atleast_2d(np.asarray(features, np.float64))
+        scores = np.array(self.predict_scores(f), np.float64)
+        for i in range(f.shape[0]):
+            maxd = int(round(2.0 ** f[i, 4] - 1.0))
+            for k, st in enumerate(self.styles):
+                if too_slow(self.algo, st, maxd):
+                    scores[i, k] = np.inf
+        return list(np.argmin(scores, axis=1))
 
     def evaluate(self, features: np.ndarray, times: np.ndarray, default_style: int) -> dict:
         t = np.asarray(times, np.float64)
@@ -175,3 +189,20 @@ class Selector:
         s.net = Mlp(int(d["n_in"]), len(s.styles))
         s.net.set_params(d["params"])
         return s
+
+
+MODEL_DIR = os.path.join(os.path.dirname(__file__), "selector_models")
+_loaded = {}
+
+
+def pick_style(graph, algo: str, model_dir: Optional[str] = None) -> str:
+    """Style name the trained selector picks for `graph` (a Graph) and `algo`
+    ("bfs", "sssp", "cc", "mis", "pr" or "tc"), from the models shipped in
+    selector_models/ unless model_dir names another directory.  Needs the GPU
+    library (the MLP forward pass runs on the device)."""
+    d = model_dir or MODEL_DIR
+    key = (d, algo)
+    if key not in _loaded:
+        _loaded[key] = Selector.load(os.path.join(d, f"selector_{algo}.npz"))
+    sel = _loaded[key]
+    return sel.styles[sel.pick(graph.features())[0]]

@@ -3,8 +3,8 @@ evaluate them on held-out graphs.
 
   python -m indigo2suite_amd.train_selector collect --count N --seed-base S \
         [--log-n LO HI] [--deg LO HI] --out FILE.npz
-  python -m indigo2suite_amd.train_selector fit --train FILE.npz --out-dir DIR
-  python -m indigo2suite_amd.train_selector eval --test FILE.npz --model-dir DIR [--json OUT]
+  python -m indigo2suite_amd.train_selector fit --train FILE.npz [FILE.npz ...] --out-dir DIR
+  python -m indigo2suite_amd.train_selector eval --test FILE.npz [FILE.npz ...] --model-dir DIR [--json OUT]
 
 `collect` runs the exhaustive autotuner (every registered style of every
 algorithm, median of three runs) on seeded graphs from four families;
@@ -32,13 +32,24 @@ def cmd_collect(a):
     t0 = time.time()
     data = autotune.collect(a.count, ALGOS, seed_base=a.seed_base, log_n_range=tuple(a.log_n),
                             deg_range=tuple(a.deg), repeats=a.repeats,
-                            log=lambda s: print(s, flush=True))
+                            log=lambda s: print(s, flush=True),
+                            checkpoint=lambda d: autotune.save_dataset(a.out, d))
     autotune.save_dataset(a.out, data)
     print(f"wrote {a.out}: {a.count} graphs in {time.time() - t0:.0f}s", flush=True)
 
 
+def load_many(paths):
+    """Concatenate datasets collected with the same style registry."""
+    parts = [autotune.load_dataset(p) for p in paths]
+    styles = parts[0]["styles"]
+    for d in parts[1:]:
+        assert d["styles"] == styles, "datasets from different style registries"
+    return {"features": np.concatenate([d["features"] for d in parts]),
+            "times": {a: np.concatenate([d["times"][a] for d in parts]) for a in styles}, "styles": styles}
+
+
 def cmd_fit(a):
-    data = autotune.load_dataset(a.train)
+    data = load_many(a.train)
     os.makedirs(a.out_dir, exist_ok=True)
     summary = {}
     for algo in ALGOS:
@@ -54,7 +65,7 @@ def cmd_fit(a):
 
 
 def cmd_eval(a):
-    data = autotune.load_dataset(a.test)
+    data = load_many(a.test)
     device.init()
     out = {}
     for algo in ALGOS:
@@ -86,14 +97,14 @@ def main(argv=None):
     c.add_argument("--repeats", type=int, default=3)
     c.add_argument("--out", required=True)
     f = sub.add_parser("fit")
-    f.add_argument("--train", required=True)
+    f.add_argument("--train", required=True, nargs="+")
     f.add_argument("--out-dir", required=True)
     f.add_argument("--epochs", type=int, default=300)
     f.add_argument("--batch", type=int, default=64)
     f.add_argument("--lr", type=float, default=2e-3)
     f.add_argument("--seed", type=int, default=1)
     e = sub.add_parser("eval")
-    e.add_argument("--test", required=True)
+    e.add_argument("--test", required=True, nargs="+")
     e.add_argument("--model-dir", required=True)
     e.add_argument("--json")
     a = ap.parse_args(argv)

@@ -1,0 +1,60 @@
+"""Shipped selector models and timing datasets: their style lists match the
+library's registry (CPU), and on the held-out graphs of 2^19 to 2^20.5
+vertices the trained picks stay close to the exhaustive best and are never
+slower than the default style on average (GPU: the MLP runs on the device)."""
+import os
+
+import numpy as np
+import pytest
+
+from indigo2suite_amd import device as D
+from indigo2suite_amd import selector as S
+
+ALGOS = ("bfs", "sssp", "cc", "mis", "pr", "tc")
+MD = S.MODEL_DIR
+
+
+def test_models_and_datasets_match_registry():
+    for algo in ALGOS:
+        m = np.load(os.path.join(MD, f"selector_{algo}.npz"))
+        assert [str(x) for x in m["styles"]] == D.styles(algo)
+        assert m["mean"].shape == (16,) and np.all(m["std"] > 0)
+    for name in sorted(os.listdir(MD)):
+        if not name.startswith(("train_", "test_")):
+            continue
+        d = np.load(os.path.join(MD, name))
+        n = d["features"].shape[0]
+        assert d["features"].shape == (n, 16) and n > 0
+        for algo in ALGOS:
+            t = d[f"times_{algo}"]
+            assert t.shape == (n, len(D.styles(algo)))
+            assert [str(x) for x in d[f"styles_{algo}"]] == D.styles(algo)
+            # every graph has at least one measured style
+            assert np.all(np.any(np.isfinite(t) & (t > 0), axis=1))
+
+
+@pytest.mark.gpu
+def test_shipped_models_on_held_out_graphs(gpu):
+    d = np.load(os.path.join(MD, "test_900000.npz"))
+    for algo in ALGOS:
+        sel = S.Selector.load(os.path.join(MD, f"selector_{algo}.npz"))
+        try:
+            dflt = D.default_entry_style(algo)
+            ev = sel.evaluate(d["features"], d[f"times_{algo}"], dflt)
+            assert ev["geomean_speedup_vs_default"] >= 0.99, (algo, ev)
+            assert ev["frac_within_10pct"] >= 0.8, (algo, ev)
+            assert ev["mean_regret"] <= (0.2 if algo == "cc" else 0.1), (algo, ev)
+        finally:
+            sel.net.free()
+
+
+@pytest.mark.gpu
+def test_pick_style_on_a_graph(gpu):
+    from indigo2suite_amd.autotune import make_graph
+    g = make_graph("rmat", 12, 8, 5, np.random.default_rng(5))
+    for algo in ALGOS:
+        st = S.pick_style(g, algo)
+        assert st in D.styles(algo)
+    for sel in S._loaded.values():
+        sel.net.free()
+    S._loaded.clear()
