@@ -135,13 +135,22 @@ int device_sp(const igx_graph* g, const int** d_sp)
 }
 
 // ---------------------------------------------------- long-row chunk list
-// No atomics, balanced however the long rows are placed: big_rows_tiles
-// counts the chunks of each 64-row tile (one wave per tile), a device-wide
-// exclusive scan turns the counts into offsets, and big_rows_fill gives every
-// wave an equal slice of the output: it finds the tile holding its first
-// chunk by binary search over the offsets and writes its slice tile by tile
-// (R-MAT's first tile alone holds thousands of hub chunks; a wave per tile
-// would write them alone).
+// No atomics, balanced however the long rows are placed, in two launches.
+// big_rows_count sums the chunks of every range of kRangeRows consecutive
+// rows (one workgroup per range).  The work of writing the list is cut into
+// units: a range with c chunks is ceil(c / kUnitChunks) units (ranges without
+// long rows are none), and big_rows_fill hands the units to its workgroups
+// round robin.  For a unit, a workgroup stages the range's row starts and
+// chunk prefixes in LDS with one coalesced pass and writes up to kUnitChunks
+// entries with 256 lanes; R-MAT's first range alone holds thousands of hub
+// chunks, which this way spread over many workgroups.  Each workgroup scans
+// the range counts itself (up to kLdsRanges ranges, in LDS); larger graphs
+// get the two prefixes from device-wide scans instead.
+constexpr int kRangeRows = 2048;
+constexpr int kRowsPerLane = kRangeRows / kBlock;
+constexpr int kLdsRanges = 4096;
+constexpr int kUnitChunks = 1024;
+
 __device__ __forceinline__ int long_row_chunks(const int* __restrict__ nindex, int n, int v)
 {
   if (v >= n) return 0;
@@ -149,94 +158,185 @@ __device__ __forceinline__ int long_row_chunks(const int* __restrict__ nindex, i
   return (d > kChunk) ? chunks_of(d) : 0;
 }
 
-__global__ void __launch_bounds__(kBlock) big_rows_tiles(const int* __restrict__ nindex, int n,
-                                                          int* __restrict__ tile_count)
+__device__ __forceinline__ int range_units(int c) { return (c + kUnitChunks - 1) / kUnitChunks; }
+
+// range_count[r] = chunks of rows [r * kRangeRows, (r + 1) * kRangeRows) and,
+// when range_units is given, range_units[r] = its units; entry nranges of
+// both is 0, so that an exclusive scan's last entry is the total
+__global__ void __launch_bounds__(kBlock) big_rows_count(const int* __restrict__ nindex, int n, int nranges,
+                                                          int* __restrict__ range_count, int* __restrict__ range_unit)
 {
-  const int ntiles = (n + kWave - 1) / kWave;
-  const int nwaves = gridDim.x * kWavesPerBlock;
-  for (int t = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; t < ntiles; t += nwaves) {
-    const int c = wave_sum(long_row_chunks(nindex, n, t * kWave + lane_id()));
-    if (lane_id() == 0) tile_count[t] = c;
+  __shared__ int lds[kWavesPerBlock];
+  for (int r = blockIdx.x; r < nranges; r += gridDim.x) {  // uniform per block
+    const long long base = (long long)r * kRangeRows;
+    int c = 0;
+    #pragma unroll
+    for (int i = 0; i < kRowsPerLane; i++) {
+      const long long v = base + i * kBlock + threadIdx.x;
+      c += (v < n) ? long_row_chunks(nindex, n, (int)v) : 0;
+    }
+    int tot = 0;
+    block_excl_scan(c, lds, &tot);
+    if (threadIdx.x == 0) {
+      range_count[r] = tot;
+      if (range_unit) range_unit[r] = range_units(tot);
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) tile_count[ntiles] = 0;  // so the scan's last offset is the total
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    range_count[nranges] = 0;
+    if (range_unit) range_unit[nranges] = 0;
+  }
 }
 
-__global__ void __launch_bounds__(kBlock) big_rows_fill(const int* __restrict__ nindex, int n,
-                                                         const int* __restrict__ tile_off, Entry* __restrict__ bigq)
+// kLdsOff: range_a holds the nranges range counts (nranges <= kLdsRanges) and
+// the prefixes are built in dynamic LDS (2 * (nranges + 1) ints); otherwise
+// range_a / range_b hold the exclusive prefixes of the chunk and unit counts,
+// nranges + 1 entries each, the last one the total.  Writes the total chunk
+// count to *d_count.
+template <bool kLdsOff>
+__global__ void __launch_bounds__(kBlock) big_rows_fill(const int* __restrict__ nindex, int n, int nranges,
+                                                         const int* __restrict__ range_a,
+                                                         const int* __restrict__ range_b, Entry* __restrict__ bigq,
+                                                         int* __restrict__ d_count)
 {
-  __shared__ int s_pre[kWavesPerBlock][kWave];
-  __shared__ int s_beg[kWavesPerBlock][kWave];
-  __shared__ int s_end[kWavesPerBlock][kWave];
-  const int w = threadIdx.x / kWave;
-  const int lane = lane_id();
-  int* pre = s_pre[w];
-  int* rbeg = s_beg[w];
-  int* rend = s_end[w];
-  const int ntiles = (n + kWave - 1) / kWave;
-  const int nwaves = gridDim.x * kWavesPerBlock;
-  const int gw = blockIdx.x * kWavesPerBlock + w;
-  const int total = tile_off[ntiles];
-  const int per = (total + nwaves - 1) / nwaves;
-  const long long j0l = (long long)gw * per;
-  if (per == 0 || j0l >= total) return;  // uniform per wave
-  const int j0 = (int)j0l;
-  const int j1 = (total - j0 < per) ? total : j0 + per;
-  for (int j = j0; j < j1;) {  // uniform per wave
-    // the tile holding chunk j, the last t with tile_off[t] <= j, by a 64-way
-    // search: each step the lanes probe 64 evenly spaced tiles of the current
-    // range (three dependent loads for 2^18 tiles instead of eighteen)
-    int lo = 0, span = ntiles;
-    while (span > 1) {
-      const int step = (span + kWave - 1) / kWave;
-      const int idx = lo + lane * step;
-      const bool le = (lane * step < span) && idx < ntiles && tile_off[idx] <= j;
-      const uint64_t m = ballot(le);  // lane 0 always holds (tile_off[lo] <= j)
-      lo += (63 - __clzll((long long)m)) * step;
-      span = step;
+  extern __shared__ int s_dyn[];
+  __shared__ int s_beg[kRangeRows + 1];
+  __shared__ int s_pre[kRangeRows];
+  __shared__ int s_red[kWavesPerBlock];
+  const int tid = threadIdx.x;
+  const int* off = range_a;
+  const int* uoff = range_b;
+  if (kLdsOff) {
+    // coalesced copy of the counts, then each wave scans a contiguous quarter
+    // 64 entries at a time and adds the quarters before it
+    int* so = s_dyn;
+    int* su = s_dyn + nranges + 1;
+    for (int r = tid; r < nranges; r += kBlock) so[r] = range_a[r];
+    __syncthreads();
+    const int w = tid / kWave, lane = lane_id();
+    const int quarter = (nranges + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int qb = w * quarter, qe = (qb + quarter < nranges) ? qb + quarter : nranges;
+    int cc = 0, cu = 0;
+    for (int b = qb; b < qe; b += kWave) {  // uniform per wave
+      const int r = b + lane;
+      const int c = (r < qe) ? so[r] : 0, u = range_units(c);
+      const int ic = wave_incl_scan(c), iu = wave_incl_scan(u);
+      if (r < qe) {
+        so[r] = cc + ic - c;
+        su[r] = cu + iu - u;
+      }
+      cc += wave_bcast(ic, kWave - 1);
+      cu += wave_bcast(iu, kWave - 1);
     }
-    const int t = lo;
-    const int t0 = tile_off[t], t1 = tile_off[t + 1];
-    const int v = t * kWave + lane;
-    const int c = long_row_chunks(nindex, n, v);
-    const int incl = wave_incl_scan(c);
-    __builtin_amdgcn_wave_barrier();
-    pre[lane] = incl - c;
-    rbeg[lane] = (c > 0) ? nindex[v] : 0;
-    rend[lane] = (c > 0) ? nindex[v + 1] : 0;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    const int b = ((j1 < t1) ? j1 : t1) - t0;  // this wave's part of the tile: [j - t0, b)
-    for (int q = j - t0 + lane; q < b; q += kWave) {
-      int l = 0, h = kWave;  // last row k with pre[k] <= q
+    __shared__ int s_wc[kWavesPerBlock], s_wu[kWavesPerBlock];
+    if (lane == 0) {
+      s_wc[w] = cc;
+      s_wu[w] = cu;
+    }
+    __syncthreads();
+    int oc = 0, ou = 0, tc = 0, tu = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      if (k < w) {
+        oc += s_wc[k];
+        ou += s_wu[k];
+      }
+      tc += s_wc[k];
+      tu += s_wu[k];
+    }
+    for (int r = qb + lane; r < qe; r += kWave) {
+      so[r] += oc;
+      su[r] += ou;
+    }
+    if (tid == 0) {
+      so[nranges] = tc;
+      su[nranges] = tu;
+    }
+    __syncthreads();
+    off = so;
+    uoff = su;
+  }
+  const int total = off[nranges];
+  const int units = uoff[nranges];
+  if (blockIdx.x == 0 && tid == 0) *d_count = total;
+  for (int j = blockIdx.x; j < units; j += gridDim.x) {  // uniform per block
+    int lo = 0, hi = nranges;  // the range of unit j: last r with uoff[r] <= j
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (uoff[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    const int r = lo;
+    const int rb = off[r], c = off[r + 1] - rb;
+    const int k = j - uoff[r];
+    const long long base = (long long)r * kRangeRows;
+    #pragma unroll
+    for (int i = 0; i < kRowsPerLane; i++) {
+      const long long v = base + i * kBlock + tid;
+      s_beg[i * kBlock + tid] = nindex[(v < n) ? v : n];
+    }
+    if (tid == 0) s_beg[kRangeRows] = nindex[(base + kRangeRows < n) ? base + kRangeRows : n];
+    __syncthreads();
+    const int row0 = tid * kRowsPerLane;
+    int cr[kRowsPerLane];
+    int local = 0;
+    #pragma unroll
+    for (int i = 0; i < kRowsPerLane; i++) {
+      const int d = (base + row0 + i < n) ? s_beg[row0 + i + 1] - s_beg[row0 + i] : 0;
+      cr[i] = (d > kChunk) ? chunks_of(d) : 0;
+      local += cr[i];
+    }
+    int tot = 0;
+    int run = block_excl_scan(local, s_red, &tot);
+    #pragma unroll
+    for (int i = 0; i < kRowsPerLane; i++) {
+      s_pre[row0 + i] = run;
+      run += cr[i];
+    }
+    __syncthreads();
+    const int qe = (c < (k + 1) * kUnitChunks) ? c : (k + 1) * kUnitChunks;
+    for (int q = k * kUnitChunks + tid; q < qe; q += kBlock) {
+      int l = 0, h = kRangeRows;  // last row with s_pre <= q (rows without chunks share the next prefix)
       while (h - l > 1) {
         const int mid = (l + h) >> 1;
-        if (pre[mid] <= q) l = mid;
+        if (s_pre[mid] <= q) l = mid;
         else h = mid;
       }
-      const int rb = rbeg[l], re = rend[l];
-      const int beg = rb + (q - pre[l]) * kChunk;
-      bigq[t0 + q] = Entry{t * kWave + l, beg, (beg + kChunk < re) ? beg + kChunk : re, 0};
+      const int b = s_beg[l] + (q - s_pre[l]) * kChunk, e = s_beg[l + 1];
+      bigq[rb + q] = Entry{(int)(base + l), b, (b + kChunk < e) ? b + kChunk : e, 0};
     }
-    __builtin_amdgcn_wave_barrier();
-    j = t0 + b;
+    __syncthreads();
   }
 }
 
 int big_rows(const igx_graph* g, void* bigq, int* d_count)
 {
-  const int grid = g_rt.persist_blocks;
-  const int ntiles = (g->nodes + kWave - 1) / kWave;
-  const int cnt = ntiles + 1;
+  const int nranges = (int)(((long long)g->nodes + kRangeRows - 1) / kRangeRows);
+  const int cnt = nranges + 1;
   const int tsum = scan_tiles_for(cnt);
   void* pb = nullptr;
-  IGX_TRY(ws_get(WS_BLOCKS, sizeof(int) * (2 * (size_t)cnt + tsum), &pb));
-  int* tile_count = (int*)pb;
-  int* tile_off = tile_count + cnt;
-  int* tile_sum = tile_off + cnt;
-  hipLaunchKernelGGL(big_rows_tiles, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes, tile_count);
-  exclusive_scan(tile_count, tile_off, cnt, tile_sum, d_count, g_rt.stream);
-  hipLaunchKernelGGL(big_rows_fill, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes,
-                     (const int*)tile_off, (Entry*)bigq);
+  IGX_TRY(ws_get(WS_BLOCKS, sizeof(int) * (4 * (size_t)cnt + tsum + 2), &pb));
+  int* range_count = (int*)pb;
+  int* range_unit = range_count + cnt;
+  int* count_off = range_unit + cnt;
+  int* unit_off = count_off + cnt;
+  int* tile_sum = unit_off + cnt;
+  int* totals = tile_sum + tsum;
+  const int grid = g_rt.persist_blocks;
+  const int cgrid = (nranges < grid) ? ((nranges > 0) ? nranges : 1) : grid;
+  const bool lds = nranges <= kLdsRanges;
+  hipLaunchKernelGGL(big_rows_count, dim3(cgrid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes, nranges,
+                     range_count, lds ? (int*)nullptr : range_unit);
+  if (lds) {
+    hipLaunchKernelGGL(big_rows_fill<true>, dim3(grid), dim3(kBlock), sizeof(int) * 2 * (size_t)cnt, g_rt.stream,
+                       g->nindex, g->nodes, nranges, (const int*)range_count, (const int*)nullptr, (Entry*)bigq,
+                       d_count);
+  } else {
+    exclusive_scan(range_count, count_off, cnt, tile_sum, totals, g_rt.stream);
+    exclusive_scan(range_unit, unit_off, cnt, tile_sum, totals + 1, g_rt.stream);
+    hipLaunchKernelGGL(big_rows_fill<false>, dim3(grid), dim3(kBlock), 0, g_rt.stream, g->nindex, g->nodes, nranges,
+                       (const int*)count_off, (const int*)unit_off, (Entry*)bigq, d_count);
+  }
+  IGX_HIP(hipGetLastError());
   return IGX_OK;
 }
 
