@@ -50,6 +50,13 @@ __device__ __forceinline__ long long wave_sum_ll(long long v)
   return v;
 }
 
+__device__ __forceinline__ long long wave_min_ll(long long v)
+{
+  #pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v = min(v, (long long)__shfl_xor(v, d, kWave));
+  return v;
+}
+
 __device__ __forceinline__ int wave_sum(int v)
 {
   #pragma unroll
@@ -183,13 +190,15 @@ struct Ctl {
 // block in the same allocation.
 constexpr int kGroups = 8;
 constexpr int kCtlBytes = 4096;
-constexpr int kStats = 4;        // integer partials per block
+constexpr int kStats = 5;        // integer partials per block
+constexpr int kStatMin = 4;      // the one reduced by minimum instead of sum
 constexpr int kStatsD = 2;       // FP64 partials per block
 constexpr int kMaxParts = 8192;  // largest grid of a round kernel
 
 struct alignas(4096) GroupSlot {
   int qcount[2];  // queue segment sizes, indexed by round parity
-  int pad[1022];
+  int tiles;      // next unclaimed tile of the group's share (pull rounds)
+  int pad[1021];
 };
 static_assert(sizeof(Ctl) <= kCtlBytes, "control block exceeds its page");
 constexpr size_t kPartsOffset = kCtlBytes + kGroups * sizeof(GroupSlot);
@@ -210,26 +219,32 @@ __device__ __forceinline__ double* ctl_parts_d(Ctl* ctl)
 __device__ __forceinline__ int my_group() { return (int)(blockIdx.x % kGroups); }
 
 // Round kernel side: every block of a launch that did not exit on `done` calls
-// this once, with all threads, and stores its sums of up to kStats values.
-// lds: kWavesPerBlock * kStats slots.
+// this once, with all threads, and stores its sums of a..d and its minimum of
+// e (integer slots 0..3 and kStatMin).  lds: kWavesPerBlock * kStats slots.
 __device__ __forceinline__ void round_partials(Ctl* ctl, long long a, long long b, long long c, long long d,
-                                               long long* lds)
+                                               long long* lds, long long e = LLONG_MAX)
 {
   a = wave_sum_ll(a);
   b = wave_sum_ll(b);
   c = wave_sum_ll(c);
   d = wave_sum_ll(d);
+  e = wave_min_ll(e);
   const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
     lds[w * kStats + 0] = a;
     lds[w * kStats + 1] = b;
     lds[w * kStats + 2] = c;
     lds[w * kStats + 3] = d;
+    lds[w * kStats + kStatMin] = e;
   }
   __syncthreads();
   if ((int)threadIdx.x < kStats) {
-    long long t = 0;
-    for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += lds[i * kStats + threadIdx.x];
+    const bool is_min = (int)threadIdx.x == kStatMin;
+    long long t = is_min ? LLONG_MAX : 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); i++) {
+      const long long x = lds[i * kStats + threadIdx.x];
+      t = is_min ? min(t, x) : t + x;
+    }
     ctl_parts(ctl)[(size_t)threadIdx.x * kMaxParts + blockIdx.x] = t;
   }
 }
@@ -279,10 +294,11 @@ struct RoundTotals {
 
 constexpr int kFinBlock = 256;
 
-// Finalize kernel (one workgroup of kFinBlock threads): sums the partials of
-// the nparts blocks of the round kernel for the statistics in `stat_mask`
-// (bit k: integer k; bit kStats + j: FP64 j), then runs fin(ctl, totals) on
-// one thread.  Does nothing once the run is done.
+// Finalize kernel (one workgroup of kFinBlock threads): reduces the partials
+// of the nparts blocks of the round kernel for the statistics in `stat_mask`
+// (bit k: integer k, summed, or the minimum for kStatMin; bit kStats + j:
+// FP64 j, summed), then runs fin(ctl, totals) on one thread.  Does nothing
+// once the run is done.
 template <typename Fin>
 __global__ void __launch_bounds__(kFinBlock) round_finalize(Ctl* ctl, int nparts, int stat_mask, Fin fin)
 {
@@ -298,10 +314,11 @@ __global__ void __launch_bounds__(kFinBlock) round_finalize(Ctl* ctl, int nparts
       }
       continue;  // uniform
     }
+    const bool is_min = k == kStatMin;
     if (k < kStats) {
       const long long* p = ctl_parts(ctl) + (size_t)k * kMaxParts;
-      long long t = 0;
-      for (int b = threadIdx.x; b < nparts; b += kFinBlock) t += p[b];
+      long long t = is_min ? LLONG_MAX : 0;
+      for (int b = threadIdx.x; b < nparts; b += kFinBlock) t = is_min ? min(t, p[b]) : t + p[b];
       s_l[threadIdx.x] = t;
     } else {
       const double* p = ctl_parts_d(ctl) + (size_t)(k - kStats) * kMaxParts;
@@ -312,7 +329,8 @@ __global__ void __launch_bounds__(kFinBlock) round_finalize(Ctl* ctl, int nparts
     __syncthreads();
     for (int off = kFinBlock / 2; off > 0; off >>= 1) {
       if ((int)threadIdx.x < off) {
-        if (k < kStats) s_l[threadIdx.x] += s_l[threadIdx.x + off];
+        if (is_min) s_l[threadIdx.x] = min(s_l[threadIdx.x], s_l[threadIdx.x + off]);
+        else if (k < kStats) s_l[threadIdx.x] += s_l[threadIdx.x + off];
         else s_d[threadIdx.x] += s_d[threadIdx.x + off];
       }
       __syncthreads();

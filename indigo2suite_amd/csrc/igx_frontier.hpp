@@ -99,10 +99,16 @@ __device__ __forceinline__ SegPrefix segq_prefix(Ctl* ctl, int parity)
 }
 __device__ __forceinline__ Entry segq_at(const SegQ& q, const SegPrefix& sp, int idx)
 {
-  int s = 0;
+  // constant indices only, so that the prefixes stay in registers
+  int s = 0, base = 0;
   #pragma unroll
-  for (int g = 1; g < kGroups; g++) s += (idx >= sp.pre[g]) ? 1 : 0;
-  return q.base[(long long)s * q.cap + (idx - sp.pre[s])];
+  for (int g = 1; g < kGroups; g++) {
+    if (idx >= sp.pre[g]) {
+      s = g;
+      base = sp.pre[g];
+    }
+  }
+  return q.base[(long long)s * q.cap + (idx - base)];
 }
 
 // Visit every 64-vertex tile [v0, v0 + 64) that holds at least one set bit of

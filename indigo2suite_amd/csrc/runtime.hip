@@ -43,6 +43,7 @@ __global__ void ctl_reset_kernel(Ctl* ctl, int n)
     GroupSlot* gs = ctl_groups(ctl) + threadIdx.x;
     gs->qcount[0] = 0;
     gs->qcount[1] = 0;
+    gs->tiles = 0;
   }
 }
 
@@ -151,11 +152,16 @@ constexpr int kRowsPerLane = kRangeRows / kBlock;
 constexpr int kLdsRanges = 4096;
 constexpr int kUnitChunks = 1024;
 
+// a row [b, e) longer than kChunk is cut into chunks of kChunk edges
+__device__ __forceinline__ int row_chunks(int b, int e)
+{
+  return (e - b > kChunk) ? chunks_of(e - b) : 0;
+}
+
 __device__ __forceinline__ int long_row_chunks(const int* __restrict__ nindex, int n, int v)
 {
   if (v >= n) return 0;
-  const int d = nindex[v + 1] - nindex[v];
-  return (d > kChunk) ? chunks_of(d) : 0;
+  return row_chunks(nindex[v], nindex[v + 1]);
 }
 
 __device__ __forceinline__ int range_units(int c) { return (c + kUnitChunks - 1) / kUnitChunks; }
@@ -281,8 +287,7 @@ __global__ void __launch_bounds__(kBlock) big_rows_fill(const int* __restrict__ 
     int local = 0;
     #pragma unroll
     for (int i = 0; i < kRowsPerLane; i++) {
-      const int d = (base + row0 + i < n) ? s_beg[row0 + i + 1] - s_beg[row0 + i] : 0;
-      cr[i] = (d > kChunk) ? chunks_of(d) : 0;
+      cr[i] = (base + row0 + i < n) ? row_chunks(s_beg[row0 + i], s_beg[row0 + i + 1]) : 0;
       local += cr[i];
     }
     int tot = 0;

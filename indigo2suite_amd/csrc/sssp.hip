@@ -331,9 +331,12 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
 // rows that are in the frontier).
 // Pull round: every row takes the minimum of dist[u] + w over the neighbours u
 // in the frontier (the only ones that can lower it; the others were taken
-// into account when they last changed) -- rows of at most kChunk edges 64 per
-// wave through an LDS atomicMin per row, the longer ones by their chunks with
-// a global atomicMin.
+// into account when they last changed) -- runs of rows of at most kChunk
+// edges streamed by one wave per 64-row tile, the longer rows by their chunks
+// with a global atomicMin.  Rows already final are skipped: with weights >= 0
+// every value a round writes is at least the smallest distance of the
+// frontier it starts from (`settled`, the minimum the previous round lowered
+// anything to), so a row with dist <= settled cannot improve, now or later.
 // The next round pulls when the next frontier's edges exceed m / kAlphaS.  A
 // round that lowers nothing ends the run.
 //
@@ -389,6 +392,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   const int mode = ctl->mode;
   const int in_b = ctl->in_bits;
   const bool out_q = (mode == 0) && (ctl->edges_next <= kQueueEdgesS);
+  const long long settled = ctl->aux_ll[1];
   const int p = iter & 1;
   const SegQ q_in = p ? q1 : q0;
   const SegQ q_out = p ? q0 : q1;
@@ -406,6 +410,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < words; i += stride) stale[i] = 0u;
   }
   long long front_e = 0, scanned = 0, relaxed = 0, rows = 0;
+  long long low = LLONG_MAX;  // smallest distance this round lowered something to
   // first marker of u in the next frontier: counts its edges, returns its row
   auto mark = [&](int u, int& ub, int& ue) -> bool {
     const unsigned bit = 1u << (u & 31);
@@ -423,7 +428,10 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       const int u = nlist[e];
       bool first = false;
       int ub = 0, ue = 0;
-      if (dist[u] > nd && atomic_min_relaxed(&dist[u], nd) > nd) first = mark(u, ub, ue);
+      if (dist[u] > nd && atomic_min_relaxed(&dist[u], nd) > nd) {
+        low = min(low, (long long)nd);
+        first = mark(u, ub, ue);
+      }
       if (out_q) push_vertex_chunks(q_mine, out_count, first && ue > ub, u, ub, ue);
     };
     if (!in_b) {
@@ -473,80 +481,6 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
                        relax);
     }
   } else {
-    // 64-row tiles.  Consecutive rows of at most kChunk edges form runs whose
-    // edges are contiguous in nlist; a run is streamed in slices of kSlice
-    // edges: the wave loads kSlice / 64 edges per lane with independent loads
-    // (ids, frontier bits, distances, weights), parks the candidates in LDS,
-    // and each lane then folds its own row's part of the slice.
-    int* cbuf = s_cand[w];
-    for (int v0 = gw * kWave; v0 < n; v0 += nwaves * kWave) {
-      const int v = v0 + lane;
-      int rb = 0, re = 0;
-      if (v < n) {
-        rb = nindex[v];
-        re = nindex[v + 1];
-        rows += 1;
-      }
-      const bool small = (v < n) && (re - rb <= kChunk);
-      int acc = kInf;
-      uint64_t todo = ballot(small);
-      while (todo != 0) {  // uniform: one run of consecutive small rows
-        const int first = __ffsll((unsigned long long)todo) - 1;
-        const uint64_t from = todo >> first;
-        const int cnt = (~from == 0) ? (kWave - first) : (__ffsll((unsigned long long)~from) - 1);
-        const int last = first + cnt - 1;
-        todo &= (cnt + first >= kWave) ? ((1ull << first) - 1) : ~(((1ull << cnt) - 1) << first);
-        const int r0 = __shfl(rb, first, kWave), r1 = __shfl(re, last, kWave);
-        const bool mine = lane >= first && lane <= last;
-        if (lane == 0) scanned += r1 - r0;
-        for (int c0 = r0; c0 < r1; c0 += kSlice) {
-          const int c1 = (r1 - c0 < kSlice) ? r1 : c0 + kSlice;
-          int u[kSlice / kWave];
-          #pragma unroll
-          for (int k = 0; k < kSlice / kWave; k++) {
-            const int e = c0 + k * kWave + lane;
-            u[k] = (e < c1) ? nlist[e] : -1;
-          }
-          bool hit[kSlice / kWave];
-          #pragma unroll
-          for (int k = 0; k < kSlice / kWave; k++) hit[k] = u[k] >= 0 && bit_of(cur, u[k]);
-          int du[kSlice / kWave], wt[kSlice / kWave];
-          #pragma unroll
-          for (int k = 0; k < kSlice / kWave; k++) {
-            const int e = c0 + k * kWave + lane;
-            du[k] = hit[k] ? dist[u[k]] : kInf;
-            wt[k] = hit[k] ? ew[e] : 0;
-          }
-          __builtin_amdgcn_wave_barrier();
-          #pragma unroll
-          for (int k = 0; k < kSlice / kWave; k++) {
-            relaxed += hit[k] ? 1 : 0;
-            cbuf[k * kWave + lane] = (du[k] != kInf) ? du[k] + wt[k] : kInf;
-          }
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-          if (mine) {
-            const int lo = (rb > c0) ? rb : c0;
-            const int hi = (re < c1) ? re : c1;
-            for (int i = lo; i < hi; i++) acc = min(acc, cbuf[i - c0]);
-          }
-          __builtin_amdgcn_wave_barrier();
-        }
-      }
-      const int len = re - rb;
-      const bool improved = small && len > 0 && acc < dist[v];
-      if (improved) {
-        dist[v] = acc;
-        front_e += len;
-      }
-      // this wave owns the tile's bits except those of rows longer than
-      // kChunk, which the chunk walk below may set concurrently
-      const uint64_t mask = ballot(improved);
-      if (mask != 0) {
-        if (lane == 0 && (unsigned)mask != 0u) atomicOr(&next[v0 >> 5], (unsigned)mask);
-        if (lane == 1 && (unsigned)(mask >> 32) != 0u) atomicOr(&next[(v0 >> 5) + 1], (unsigned)(mask >> 32));
-      }
-    }
     // rows longer than kChunk: one wave per chunk, two chunks per step, their
     // 2 x kChunk edges loaded together, a wave minimum per chunk, and one
     // atomicMin on each row's distance
@@ -559,6 +493,10 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       for (int h = 0; h < kPair; h++) {
         const int ci = c0 + h * nwaves;
         en[h] = (ci < nbig) ? bigq[ci] : Entry{0, 0, 0, 0};
+      }
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {  // chunks of final rows are skipped
+        if (en[h].end > en[h].beg && dist[en[h].v] <= settled) en[h].end = en[h].beg;
       }
       int u[kPair][kPer];
       #pragma unroll
@@ -575,17 +513,20 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         #pragma unroll
         for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && bit_of(cur, u[h][k]);
       }
+      int du[kPair][kPer];
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {
+        #pragma unroll
+        for (int k = 0; k < kPer; k++) du[h][k] = hit[h][k] ? dist[u[h][k]] : kInf;
+      }
       int cand[kPair];
       #pragma unroll
       for (int h = 0; h < kPair; h++) {
         cand[h] = kInf;
         #pragma unroll
         for (int k = 0; k < kPer; k++) {
-          if (hit[h][k]) {
-            relaxed += 1;
-            const int du = dist[u[h][k]];
-            if (du != kInf) cand[h] = min(cand[h], du + ew[en[h].beg + k * kWave + lane]);
-          }
+          relaxed += hit[h][k] ? 1 : 0;
+          if (du[h][k] != kInf) cand[h] = min(cand[h], du[h][k] + ew[en[h].beg + k * kWave + lane]);
         }
       }
       #pragma unroll
@@ -600,13 +541,107 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
           const int v = en[h].v;
           if (en[h].end > en[h].beg && cand[h] < dist[v] && atomic_min_relaxed(&dist[v], cand[h]) > cand[h]) {
             int ub, ue;
+            low = min(low, (long long)cand[h]);
             mark(v, ub, ue);
           }
         }
       }
     }
+    // 64-row tiles.  Consecutive rows of at most kChunk edges form runs whose
+    // edges are contiguous in nlist; a run is streamed in slices of kSlice
+    // edges: the wave loads kSlice / 64 edges per lane with independent loads
+    // (ids, frontier bits, distances, weights), parks the candidates in LDS,
+    // and each lane then folds its own row's part of the slice.
+    // The tiles are claimed dynamically, two at a time, from the block's
+    // group's eighth of them (tile weights vary by more than 4x on R-MAT,
+    // the long rows' chunks above by little), so waves finish together.
+    int* cbuf = s_cand[w];
+    const int ntiles = (n + kWave - 1) / kWave;
+    const int grp = my_group();
+    const int t_end = (int)((long long)ntiles * (grp + 1) / kGroups);
+    const int t_beg = (int)((long long)ntiles * grp / kGroups);
+    int* t_next = &ctl_groups(ctl)[grp].tiles;
+    constexpr int kGrab = 2;
+    for (;;) {  // uniform per wave
+      int t0 = 0;
+      if (lane == 0) t0 = atomicAdd(t_next, kGrab);
+      t0 = wave_bcast(t0, 0) + t_beg;
+      if (t0 >= t_end) break;
+      for (int t = t0; t < t0 + kGrab && t < t_end; t++) {
+        const int v0 = t * kWave;
+        const int v = v0 + lane;
+        int rb = 0, re = 0, dv = 0;
+        if (v < n) {
+          rb = nindex[v];
+          re = nindex[v + 1];
+          dv = dist[v];
+          rows += 1;
+        }
+        // rows at most `settled` are final (see above) and skip their edges
+        const bool small = (v < n) && (re - rb <= kChunk) && dv > settled;
+        int acc = kInf;
+        uint64_t todo = ballot(small);
+        while (todo != 0) {  // uniform: one run of consecutive small rows
+          const int first = __ffsll((unsigned long long)todo) - 1;
+          const uint64_t from = todo >> first;
+          const int cnt = (~from == 0) ? (kWave - first) : (__ffsll((unsigned long long)~from) - 1);
+          const int last = first + cnt - 1;
+          todo &= (cnt + first >= kWave) ? ((1ull << first) - 1) : ~(((1ull << cnt) - 1) << first);
+          const int r0 = __shfl(rb, first, kWave), r1 = __shfl(re, last, kWave);
+          const bool mine = lane >= first && lane <= last;
+          if (lane == 0) scanned += r1 - r0;
+          constexpr int kPerLane = kSlice / kWave;
+          for (int c0 = r0; c0 < r1; c0 += kSlice) {
+            const int c1 = (r1 - c0 < kSlice) ? r1 : c0 + kSlice;
+            int u[kPerLane];
+            #pragma unroll
+            for (int k = 0; k < kPerLane; k++) {
+              const int e = c0 + k * kWave + lane;
+              u[k] = (e < c1) ? nlist[e] : -1;
+            }
+            bool hit[kPerLane];
+            #pragma unroll
+            for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0 && bit_of(cur, u[k]);
+            int du[kPerLane], wt[kPerLane];
+            #pragma unroll
+            for (int k = 0; k < kPerLane; k++) {
+              du[k] = hit[k] ? dist[u[k]] : kInf;
+              wt[k] = hit[k] ? ew[c0 + k * kWave + lane] : 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            #pragma unroll
+            for (int k = 0; k < kPerLane; k++) {
+              relaxed += hit[k] ? 1 : 0;
+              cbuf[k * kWave + lane] = (du[k] != kInf) ? du[k] + wt[k] : kInf;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (mine) {
+              const int lo = (rb > c0) ? rb : c0;
+              const int hi = (re < c1) ? re : c1;
+              for (int i = lo; i < hi; i++) acc = min(acc, cbuf[i - c0]);
+            }
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
+        const int len = re - rb;
+        const bool improved = small && len > 0 && acc < dv;
+        if (improved) {
+          dist[v] = acc;
+          front_e += len;
+          low = min(low, (long long)acc);
+        }
+        // this wave owns the tile's bits except those of rows longer than
+        // kChunk, which other waves' chunk walks may set concurrently
+        const uint64_t mask = ballot(improved);
+        if (mask != 0) {
+          if (lane == 0 && (unsigned)mask != 0u) atomicOr(&next[v0 >> 5], (unsigned)mask);
+          if (lane == 1 && (unsigned)(mask >> 32) != 0u) atomicOr(&next[(v0 >> 5) + 1], (unsigned)(mask >> 32));
+        }
+      }
+    }
   }
-  round_partials(ctl, front_e, scanned, relaxed, rows, s_red);
+  round_partials(ctl, front_e, scanned, relaxed, rows, s_red, low);
 }
 
 // Round decision (run by round_finalize after each sssp_do_round).
@@ -620,8 +655,12 @@ struct FinSsspDo {
     ctl->work_edges += t.s[kStatScanned];
     ctl->aux_ll[0] += t.s[kStatRelaxed];
     ctl->work_vertices += t.s[kStatRows];
+    ctl->aux_ll[1] = t.s[kStatMin];  // the next frontier's smallest distance
     GroupSlot* gs = ctl_groups(ctl);
-    for (int g = 0; g < kGroups; g++) gs[g].qcount[p] = 0;
+    for (int g = 0; g < kGroups; g++) {
+      gs[g].qcount[p] = 0;
+      gs[g].tiles = 0;
+    }
     ctl->iter += 1;
     ctl->in_bits = out_q ? 0 : 1;
     ctl->edges_next = ef;
@@ -798,7 +837,7 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
                            g->eweight, g->nodes, (long long)g->edges, dist, q0, q1, (const Entry*)pb,
                            (const int*)pc, (unsigned*)pm, words, R.d_ctl);
         ktimer_end("sssp_do_round");
-        launch_finalize(R.persist_blocks, 0xF, FinSsspDo{(long long)g->edges});
+        launch_finalize(R.persist_blocks, 0xF | (1 << kStatMin), FinSsspDo{(long long)g->edges});
       },
       sssp_round_cap(g), &rounds, &launches, 8);
   IGX_TRY(timer_stop(&info->seconds));
