@@ -552,23 +552,24 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     // edges: the wave loads kSlice / 64 edges per lane with independent loads
     // (ids, frontier bits, distances, weights), parks the candidates in LDS,
     // and each lane then folds its own row's part of the slice.
-    // The tiles are claimed dynamically, two at a time, from the block's
-    // group's eighth of them (tile weights vary by more than 4x on R-MAT,
-    // the long rows' chunks above by little), so waves finish together.
+    // The tiles are claimed dynamically, two at a time: the blocks of group g
+    // take the tiles g, g + kGroups, g + 2 kGroups, ... in turn (tile weights
+    // vary by more than 4x on R-MAT and drift with the vertex id; the long
+    // rows' chunks above by little), so waves finish together.
     int* cbuf = s_cand[w];
     const int ntiles = (n + kWave - 1) / kWave;
     const int grp = my_group();
-    const int t_end = (int)((long long)ntiles * (grp + 1) / kGroups);
-    const int t_beg = (int)((long long)ntiles * grp / kGroups);
     int* t_next = &ctl_groups(ctl)[grp].tiles;
     constexpr int kGrab = 2;
     for (;;) {  // uniform per wave
-      int t0 = 0;
-      if (lane == 0) t0 = atomicAdd(t_next, kGrab);
-      t0 = wave_bcast(t0, 0) + t_beg;
-      if (t0 >= t_end) break;
-      for (int t = t0; t < t0 + kGrab && t < t_end; t++) {
-        const int v0 = t * kWave;
+      int k0 = 0;
+      if (lane == 0) k0 = atomicAdd(t_next, kGrab);
+      k0 = wave_bcast(k0, 0);
+      if (grp + (long long)k0 * kGroups >= ntiles) break;
+      for (int kc = k0; kc < k0 + kGrab; kc++) {
+        const long long tl = grp + (long long)kc * kGroups;
+        if (tl >= ntiles) break;  // uniform
+        const int v0 = (int)tl * kWave;
         const int v = v0 + lane;
         int rb = 0, re = 0, dv = 0;
         if (v < n) {

@@ -206,119 +206,133 @@ __global__ void __launch_bounds__(kBlock) tc_orient_fill(const int* __restrict__
 // the power of two >= 2 |N+(v)|, multiplicative hashing, linear probing), then
 // walks the concatenated out-lists of v's out-neighbours (wave merge path, so
 // short and long lists share the lanes evenly), kIlp elements per lane per
-// step with their loads issued together, and probes each element once.
-// Out-lists longer than kHashMax / 2 fall back to a per-lane binary search of
-// N+(v) in memory.  Launched with several workgroups per CU slot so that more
-// waves hide the latency of the probe chain.
+// step with their loads issued together, and probes each element once.  An
+// out-list longer than kHashMax / 2 is hashed in passes over consecutive
+// slices of it (the list is sorted, so a slice is a value range and elements
+// outside it skip the probe).
+// Vertices are claimed dynamically in units of kClaim consecutive ones; the
+// blocks of group g take the units g, g + kGroups, ... in turn: the probe
+// work per vertex varies by 50x on R-MAT (and drifts with the id), and a
+// fixed assignment left some waves with 20x the mean.  Launched with
+// several workgroups per CU slot so that more waves hide the latency of the
+// probe chain.
 constexpr int kHashMax = 1024;  // hash slots per wave
 constexpr int kIlp = 4;
+constexpr int kClaim = 4;
+constexpr int kClaimStride = 64;  // ints between the groups' claim counters
 
 __device__ __forceinline__ unsigned tc_hash(int x, int bits)
 {
   return ((unsigned)x * 0x9E3779B1u) >> (32 - bits);
 }
 
+__global__ void __launch_bounds__(kBlock) tc_clear(int* __restrict__ p, int count)
+{
+  for (int i = threadIdx.x; i < count; i += kBlock) p[i] = 0;
+}
+
 __global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restrict__ onidx,
-                                                             const int* __restrict__ olist, int n, unsigned mult,
+                                                             const int* __restrict__ olist, int n,
+                                                             int* __restrict__ claim,
                                                              unsigned long long* __restrict__ count)
 {
   __shared__ int s_tab[kWavesPerBlock][kHashMax];
   __shared__ int s_excl[kWavesPerBlock][kWave];
   __shared__ int s_beg[kWavesPerBlock][kWave];
   const int w = threadIdx.x / kWave;
-  const int waves = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
   int* tab = s_tab[w];
   int* excl = s_excl[w];
   int* ebeg = s_beg[w];
+  const int grp = (int)(blockIdx.x % kGroups);
+  int* next = claim + grp * kClaimStride;
+  const long long units = ((long long)n + kClaim - 1) / kClaim;
   unsigned long long c = 0;
-  for (int k = blockIdx.x * kWavesPerBlock + w; k < n; k += waves) {  // uniform per wave
-    const int v = spread_vertex(k, n, mult);
-    const int beg = onidx[v], end = onidx[v + 1];
-    const int len = end - beg;
-    if (len < 2) continue;
-    if (2 * len > kHashMax) {
-      for (int j = beg + lane; j < end; j += kWave) {
-        const int u = olist[j];
-        for (int q = onidx[u]; q < onidx[u + 1]; q++) {
-          const int x = olist[q];
-          int lo = beg, hi = end;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (olist[mid] < x) lo = mid + 1;
-            else hi = mid;
-          }
-          c += (lo < end && olist[lo] == x) ? 1u : 0u;
+  for (;;) {  // uniform per wave
+    int k = 0;
+    if (lane == 0) k = atomicAdd(next, 1);
+    const long long unit = grp + (long long)wave_bcast(k, 0) * kGroups;
+    if (unit >= units) break;
+    const int v1 = (int)((unit + 1) * kClaim < n ? (unit + 1) * kClaim : n);
+    for (int v = (int)(unit * kClaim); v < v1; v++) {
+      const int beg = onidx[v], end = onidx[v + 1];
+      const int len = end - beg;
+      if (len < 2) continue;
+      const int passes = (2 * len + kHashMax - 1) / kHashMax;
+      for (int ps = 0; ps < passes; ps++) {
+        // the slice [pb, pe) of N+(v), at most kHashMax / 2 long
+        const int pb = beg + (int)((long long)len * ps / passes);
+        const int pe = beg + (int)((long long)len * (ps + 1) / passes);
+        const int plen = pe - pb;
+        const int lo_val = olist[pb], hi_val = olist[pe - 1];
+        int bits = 1;
+        while ((1 << bits) < 2 * plen) bits++;
+        const int size = 1 << bits;
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < size; i += kWave) tab[i] = -1;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (int i = lane; i < plen; i += kWave) {
+          const int x = olist[pb + i];
+          unsigned h = tc_hash(x, bits);
+          while (atomicCAS(&tab[h], -1, x) != -1) h = (h + 1) & (size - 1);
         }
-      }
-      continue;
-    }
-    int bits = 1;
-    while ((1 << bits) < 2 * len) bits++;
-    const int size = 1 << bits;
-    for (int i = lane; i < size; i += kWave) tab[i] = -1;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    for (int i = lane; i < len; i += kWave) {
-      const int x = olist[beg + i];
-      unsigned h = tc_hash(x, bits);
-      while (atomicCAS(&tab[h], -1, x) != -1) h = (h + 1) & (size - 1);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    for (int gb = beg; gb < end; gb += kWave) {
-      const int j = gb + lane;
-      int ob = 0, oe = 0;
-      if (j < end) {
-        const int u = olist[j];
-        ob = onidx[u];
-        oe = onidx[u + 1];
-      }
-      const int el = oe - ob;
-      const int incl = wave_incl_scan(el);
-      const int total = wave_bcast(incl, kWave - 1);
-      if (total == 0) continue;  // uniform
-      __builtin_amdgcn_wave_barrier();
-      excl[lane] = incl - el;
-      ebeg[lane] = ob;
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      for (int tb = 0; tb < total; tb += kIlp * kWave) {
-        int x[kIlp];
-        #pragma unroll
-        for (int q = 0; q < kIlp; q++) {
-          const int t = tb + q * kWave + lane;
-          int lo = 0;
-          if (t < total) {
-            int hi = kWave;  // last k with excl[k] <= t
-            while (hi - lo > 1) {
-              const int mid = (lo + hi) >> 1;
-              if (excl[mid] <= t) lo = mid;
-              else hi = mid;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (int gb = beg; gb < end; gb += kWave) {
+          const int j = gb + lane;
+          int ob = 0, oe = 0;
+          if (j < end) {
+            const int u = olist[j];
+            ob = onidx[u];
+            oe = onidx[u + 1];
+          }
+          const int el = oe - ob;
+          const int incl = wave_incl_scan(el);
+          const int total = wave_bcast(incl, kWave - 1);
+          if (total == 0) continue;  // uniform
+          __builtin_amdgcn_wave_barrier();
+          excl[lane] = incl - el;
+          ebeg[lane] = ob;
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          for (int tb = 0; tb < total; tb += kIlp * kWave) {
+            int x[kIlp];
+            #pragma unroll
+            for (int q = 0; q < kIlp; q++) {
+              const int t = tb + q * kWave + lane;
+              int lo = 0;
+              if (t < total) {
+                int hi = kWave;  // last k with excl[k] <= t
+                while (hi - lo > 1) {
+                  const int mid = (lo + hi) >> 1;
+                  if (excl[mid] <= t) lo = mid;
+                  else hi = mid;
+                }
+              }
+              x[q] = (t < total) ? ebeg[lo] + (t - excl[lo]) : -1;
+            }
+            #pragma unroll
+            for (int q = 0; q < kIlp; q++) x[q] = (x[q] >= 0) ? olist[x[q]] : -1;
+            #pragma unroll
+            for (int q = 0; q < kIlp; q++) {
+              if (x[q] < lo_val || x[q] > hi_val) continue;
+              unsigned h = tc_hash(x[q], bits);
+              while (true) {
+                const int y = tab[h];
+                if (y == x[q]) {
+                  c += 1u;
+                  break;
+                }
+                if (y < 0) break;
+                h = (h + 1) & (size - 1);
+              }
             }
           }
-          x[q] = (t < total) ? ebeg[lo] + (t - excl[lo]) : -1;
-        }
-        #pragma unroll
-        for (int q = 0; q < kIlp; q++) x[q] = (x[q] >= 0) ? olist[x[q]] : -1;
-        #pragma unroll
-        for (int q = 0; q < kIlp; q++) {
-          if (x[q] < 0) continue;
-          unsigned h = tc_hash(x[q], bits);
-          while (true) {
-            const int y = tab[h];
-            if (y == x[q]) {
-              c += 1u;
-              break;
-            }
-            if (y < 0) break;
-            h = (h + 1) & (size - 1);
-          }
+          __builtin_amdgcn_wave_barrier();
         }
       }
-      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
   }
   wave_add_count(count, c);
 }
@@ -388,6 +402,8 @@ int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info
   // that an asymmetric input cannot overrun the list
   IGX_TRY(ws_get(WS_C, sizeof(int) * ((size_t)g->edges + 64), &plist));
   IGX_TRY(ws_get(WS_E, sizeof(int) * ((size_t)scan_tiles_for(n + 1) + 64), &ptile));
+  void* pclaim = nullptr;
+  IGX_TRY(ws_get(WS_D, sizeof(int) * kGroups * kClaimStride, &pclaim));
   int* ocount = (int*)pcnt;
   int* onidx = (int*)pidx;
   int* tiles = (int*)ptile;
@@ -401,17 +417,18 @@ int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info
   exclusive_scan(ocount, onidx, n + 1, tiles, tiles + scan_tiles_for(n + 1), R.stream);
   hipLaunchKernelGGL(tc_orient_fill, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, mult,
                      onidx, (int*)plist);
+  hipLaunchKernelGGL(tc_clear, dim3(1), dim3(kBlock), 0, R.stream, (int*)pclaim, kGroups * kClaimStride);
   ktimer_begin("tc_oriented_count");
-  // identity order here: neighbouring vertices share out-lists in cache, which
-  // outweighs the imbalance the spread order removes
+  // identity order within each group's range: neighbouring vertices share
+  // out-lists in cache
   hipLaunchKernelGGL(tc_oriented_count, dim3(4 * R.persist_blocks), dim3(kBlock), 0, R.stream, onidx,
-                     (const int*)plist, n, 1u, count);
+                     (const int*)plist, n, (int*)pclaim, count);
   ktimer_end("tc_oriented_count");
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
   IGX_HIP(hipMemcpy(&info->count, count, sizeof(unsigned long long), hipMemcpyDeviceToHost));
   info->iterations = 1;
-  info->launches = 8;
+  info->launches = 9;
   return IGX_OK;
 }
 

@@ -172,3 +172,19 @@ def test_sssp_needs_weights_and_source_in_range(dev):
         assert "ERR_ARG" in str(e.value)
     finally:
         dg.free()
+
+
+@pytest.mark.parametrize("k", [600, 1100])
+def test_tc_complete_graph_closed_form(dev, k):
+    """K_k has C(k, 3) triangles; its oriented out-lists reach k - 1 > 512
+    entries, so the oriented style hashes them in slices."""
+    i, j = np.triu_indices(k, 1)
+    g = G.from_edges(k, i.astype(np.int32), j.astype(np.int32), name=f"complete{k}")
+    dg = dev.DeviceGraph(g)
+    try:
+        want = k * (k - 1) * (k - 2) // 6
+        for st in ("oriented_lds", "v_merge_wave"):
+            _say("tc", g.name, st)
+            assert dev.run("tc", dg, st).out == want, st
+    finally:
+        dg.free()
