@@ -249,6 +249,21 @@ __device__ __forceinline__ void round_partials(Ctl* ctl, long long a, long long 
   }
 }
 
+// Integer partial `slot` of this block, summed over its threads (all threads
+// call; lds: kWavesPerBlock); leaves the other slots alone.
+__device__ __forceinline__ void round_partial_ll(Ctl* ctl, int slot, long long v, long long* lds)
+{
+  v = wave_sum_ll(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += lds[i];
+    ctl_parts(ctl)[(size_t)slot * kMaxParts + blockIdx.x] = t;
+  }
+}
+
 // FP64 partial `slot` of this block (all threads call; lds: kWavesPerBlock).
 __device__ __forceinline__ void round_partial_d(Ctl* ctl, int slot, double v, double* lds)
 {

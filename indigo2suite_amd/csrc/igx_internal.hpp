@@ -160,6 +160,16 @@ int device_sp(const igx_graph* g, const int** d_sp);
 // entries.
 int big_rows(const igx_graph* g, void* bigq, int* d_count);
 
+// Tiles of 64 rows a wave claims at once in the kernels that hand out tiles
+// dynamically: about 4096 edges of work per claim (two tiles on R-MAT 22,
+// sixteen on a 2-D grid), so the shared claim counters stay cold.
+inline int tile_grab(const igx_graph* g)
+{
+  const long long tiles = ((long long)g->nodes + 63) / 64;
+  const long long grab = 4096LL * tiles / (g->edges > 0 ? (long long)g->edges : 1LL);
+  return (int)(grab < 1 ? 1 : (grab > 32 ? 32 : grab));
+}
+
 // events bracketing the timed region
 int timer_start();
 int timer_stop(double* seconds);

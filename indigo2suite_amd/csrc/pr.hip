@@ -6,7 +6,11 @@
 // Precision: scores and contributions are FP32; every neighbour sum is
 // compensated (TwoSum per addition; partial sums are merged as (sum, error)
 // pairs), and the per-iteration L1 residual that decides convergence is
-// accumulated in FP64.  Results are checked against an FP64 host oracle.
+// accumulated exactly: each row's |change| is computed in FP64 and rounded to
+// a multiple of 2^-50, and those integers are summed in 64 bits, so the
+// residual does not depend on which wave handled which row (below 1e-9 from
+// the FP64 sum for a million rows).  Results are checked against an FP64 host
+// oracle.
 //
 // Reference family: codes/cuda/pr-cuda (54 variants).
 //   pull_thread_kahan   [C-PRt] contrib:41-47 + pull:49-70 (thread per vertex)
@@ -14,9 +18,10 @@
 //   pull_block_kahan    PR_CUDA_Pull_..._Block siblings (one workgroup per vertex)
 //   pull_balanced_kahan (no reference counterpart) rows classed by degree:
 //                       small rows streamed in runs of consecutive rows and
-//                       summed in edge order by their lane, wave per medium
-//                       row, and large rows split into chunks whose
-//                       compensated partial sums are merged in chunk order
+//                       summed in edge order by their lane, longer rows split
+//                       into chunks of at most kChunk edges (one wave each)
+//                       whose compensated partial sums are merged in chunk
+//                       order
 // The contribution of the next iteration (score / degree) is written by the
 // same kernel that produces the score, so one pass per iteration remains.
 #include "igx_internal.hpp"
@@ -26,9 +31,9 @@ namespace igx {
 namespace {
 
 constexpr float kDamp = 0.85f;  // pr_cuda.h:43
-constexpr int kSmallRow = 32;    // rows streamed in runs by pr_bal_main
-constexpr int kMediumRow = 2048; // rows with one wave each; longer ones by chunks
+constexpr int kSmallRow = 32;    // rows streamed in runs by pr_bal_main; longer ones by chunks
 constexpr int kSlice = 512;      // edges per streamed slice
+constexpr double kResScale = 1125899906842624.0;  // 2^50: fixed-point unit of the residual
 
 struct Pair {
   float s;  // running sum
@@ -78,28 +83,29 @@ __global__ void __launch_bounds__(kBlock) pr_init(const int* __restrict__ nindex
   }
 }
 
-// finish one vertex: new score, next contribution, FP64 residual term
-__device__ __forceinline__ double pr_finish(int v, Pair acc, float base, const int* __restrict__ nindex,
-                                            float* __restrict__ score, float* __restrict__ contrib_next)
+// finish one vertex: new score, next contribution, residual term (|change|
+// in units of 2^-50, rounded to nearest)
+__device__ __forceinline__ long long pr_finish(int v, Pair acc, float base, const int* __restrict__ nindex,
+                                               float* __restrict__ score, float* __restrict__ contrib_next)
 {
   const float sum = acc.s + acc.e;
   const float value = base + kDamp * sum;
   const float old = score[v];
   score[v] = value;
   contrib_next[v] = value / (float)(nindex[v + 1] - nindex[v]);
-  return fabs((double)value - (double)old);
+  return (long long)(fabs((double)value - (double)old) * kResScale + 0.5);
 }
 
 
-// End of an iteration (run by round_finalize): the FP64 L1 residual is the
-// block-ordered sum of the per-block partials (slot 0: rows scored by the main
-// kernel, slot 1: rows merged from chunks), so it is the same on every run.
+// End of an iteration (run by round_finalize): the L1 residual is the exact
+// integer sum of the per-block partials (slot 0: rows scored by the main
+// kernel, slot 1: rows merged from chunks), scaled back from 2^-50 units.
 struct FinPr {
   double eps;
   int max_iter;
   __device__ void operator()(Ctl* ctl, const RoundTotals& t) const
   {
-    const double r = t.d[0] + t.d[1];
+    const double r = (double)(t.s[0] + t.s[1]) / kResScale;
     ctl->res_last = r;
     ctl->iter += 1;
     if (r < eps || ctl->iter >= max_iter) ctl->done = 1;
@@ -115,10 +121,10 @@ __global__ void __launch_bounds__(kBlock) pr_pull(const int* __restrict__ nindex
                                                    float* __restrict__ contrib_next, Ctl* ctl, double eps,
                                                    int max_iter)
 {
-  __shared__ double s_red[kWavesPerBlock];
+  __shared__ long long s_red[kWavesPerBlock];
   __shared__ Pair s_pair[kWavesPerBlock];
   if (round_done(ctl)) return;
-  double err = 0.0;
+  long long err = 0;
   if constexpr (G == kThread) {
     const int stride = gridDim.x * kBlock;
     for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
@@ -151,13 +157,16 @@ __global__ void __launch_bounds__(kBlock) pr_pull(const int* __restrict__ nindex
       __syncthreads();
     }
   }
-  round_partial_d(ctl, 0, err, s_red);
+  round_partial_ll(ctl, 0, err, s_red);
 }
 
 // ------------------------------------------------------------- balanced ----
-// Row classes are built once per run: medium rows go to a list; every large
-// row reserves a contiguous run of chunk slots (its kChunk-edge pieces in edge
-// order), so merging a row's partial sums in slot order is deterministic.
+// Row classes are built once per run: every row longer than kSmallRow is cut
+// into chunks of at most kChunk edges (its pieces in edge order, in a
+// contiguous run of chunk slots reserved with one atomic per wave).  A row of
+// one chunk is finished by the wave that sums it (Entry::pad = -1); the others
+// leave compensated partial sums that pr_bal_big merges in slot order, so the
+// result does not depend on the schedule.
 struct BigRow {
   int v;
   int first;  // first chunk slot
@@ -169,7 +178,8 @@ __global__ void pr_zero(int* __restrict__ p, int count)
   for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) pr_classify(const int* __restrict__ nindex, int n, int* __restrict__ medium,
+// counts[1]: rows of several chunks (big); counts[2]: chunks
+__global__ void __launch_bounds__(kBlock) pr_classify(const int* __restrict__ nindex, int n,
                                                        int* __restrict__ counts, Entry* __restrict__ chunks,
                                                        BigRow* __restrict__ big)
 {
@@ -182,41 +192,63 @@ __global__ void __launch_bounds__(kBlock) pr_classify(const int* __restrict__ ni
       end = nindex[v + 1];
       deg = end - beg;
     }
-    const bool is_med = deg > kSmallRow && deg <= kMediumRow;
-    const int ms = wave_push_slot(&counts[0], is_med);
-    if (is_med) medium[ms] = v;
-    const bool is_big = deg > kMediumRow;
-    const int bs = wave_push_slot(&counts[1], is_big);
-    if (is_big) {
-      const int c = chunks_of(deg);
-      const int first = atomicAdd(&counts[2], c);
-      big[bs] = BigRow{v, first, c};
-      for (int k = 0; k < c; k++) {
-        const int b = beg + k * kChunk;
-        chunks[first + k] = Entry{v, b, (b + kChunk < end) ? b + kChunk : end, 0};
-      }
+    const int c = (deg > kSmallRow) ? chunks_of(deg) : 0;
+    const int first = wave_reserve(&counts[2], c);
+    const int bs = wave_push_slot(&counts[1], c > 1);
+    if (c > 1) big[bs] = BigRow{v, first, c};
+    for (int k = 0; k < c; k++) {
+      const int b = beg + k * kChunk;
+      chunks[first + k] = Entry{v, b, (b + kChunk < end) ? b + kChunk : end, (c == 1) ? -1 : 0};
     }
   }
 }
 
+// One iteration's sums: first the chunks (one wave each, kChunk / 64 loads per
+// lane issued together; chunks are at most kChunk edges, so a fixed
+// round-robin share balances), then the small rows in 64-row tiles, also
+// round robin.  Small rows:
+// consecutive small rows form runs whose edges are contiguous, streamed kSlice
+// at a time (independent loads, values parked in LDS); each lane then adds its
+// own row's values in edge order.
 __global__ void __launch_bounds__(kBlock) pr_bal_main(const int* __restrict__ nindex, const int* __restrict__ nlist,
                                                        int n, float base, float* __restrict__ score,
                                                        const float* __restrict__ contrib_cur,
                                                        float* __restrict__ contrib_next,
-                                                       const int* __restrict__ medium, const int* __restrict__ counts,
+                                                       const int* __restrict__ counts,
                                                        const Entry* __restrict__ chunks, Pair* __restrict__ partial,
                                                        Ctl* ctl)
 {
-  __shared__ double s_red[kWavesPerBlock];
+  __shared__ long long s_red[kWavesPerBlock];
   __shared__ float s_val[kWavesPerBlock][kSlice];
   if (round_done(ctl)) return;
-  double err = 0.0;
+  long long err = 0;
   const int waves = gridDim.x * kWavesPerBlock;
   const int gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   const int lane = lane_id();
-  // small rows: 64-row tiles; consecutive small rows form runs whose edges
-  // are contiguous, streamed kSlice at a time (independent loads, values
-  // parked in LDS); each lane then adds its own row's values in edge order
+  constexpr int kPer = kChunk / kWave;
+  const int nch = counts[2];
+  for (int k = gw; k < nch; k += waves) {  // uniform per wave
+    const Entry en = chunks[k];
+    int u[kPer];
+    #pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int i = en.beg + j * kWave + lane;
+      u[j] = (i < en.end) ? nlist[i] : -1;
+    }
+    float x[kPer];
+    #pragma unroll
+    for (int j = 0; j < kPer; j++) x[j] = (u[j] >= 0) ? contrib_cur[u[j]] : 0.0f;
+    Pair acc{0.0f, 0.0f};
+    #pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (u[j] >= 0) two_sum_add(acc, x[j]);
+    }
+    acc = wave_merge(acc);
+    if (lane == 0) {
+      if (en.pad < 0) err += pr_finish(en.v, acc, base, nindex, score, contrib_next);
+      else partial[k] = acc;
+    }
+  }
   float* vbuf = s_val[threadIdx.x / kWave];
   for (int v0 = gw * kWave; v0 < n; v0 += waves * kWave) {  // uniform per wave
     const int v = v0 + lane;
@@ -238,18 +270,18 @@ __global__ void __launch_bounds__(kBlock) pr_bal_main(const int* __restrict__ ni
       const bool mine = lane >= first && lane <= last;
       for (int c0 = r0; c0 < r1; c0 += kSlice) {
         const int c1 = (r1 - c0 < kSlice) ? r1 : c0 + kSlice;
-        int u[kSlice / kWave];
+        int uu[kSlice / kWave];
         #pragma unroll
         for (int k = 0; k < kSlice / kWave; k++) {
           const int e = c0 + k * kWave + lane;
-          u[k] = (e < c1) ? nlist[e] : -1;
+          uu[k] = (e < c1) ? nlist[e] : -1;
         }
-        float x[kSlice / kWave];
+        float xx[kSlice / kWave];
         #pragma unroll
-        for (int k = 0; k < kSlice / kWave; k++) x[k] = (u[k] >= 0) ? contrib_cur[u[k]] : 0.0f;
+        for (int k = 0; k < kSlice / kWave; k++) xx[k] = (uu[k] >= 0) ? contrib_cur[uu[k]] : 0.0f;
         __builtin_amdgcn_wave_barrier();
         #pragma unroll
-        for (int k = 0; k < kSlice / kWave; k++) vbuf[k * kWave + lane] = x[k];
+        for (int k = 0; k < kSlice / kWave; k++) vbuf[k * kWave + lane] = xx[k];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         if (mine) {
@@ -262,36 +294,17 @@ __global__ void __launch_bounds__(kBlock) pr_bal_main(const int* __restrict__ ni
     }
     if (small) err += pr_finish(v, acc, base, nindex, score, contrib_next);
   }
-  // medium rows: one wave each
-  const int nmed = counts[0];
-  for (int k = gw; k < nmed; k += waves) {
-    const int v = medium[k];
-    Pair acc{0.0f, 0.0f};
-    for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) two_sum_add(acc, contrib_cur[nlist[i]]);
-    acc = wave_merge(acc);
-    if (lane == 0) err += pr_finish(v, acc, base, nindex, score, contrib_next);
-  }
-  // chunks of large rows: one wave each, partial pairs to memory
-  const int nch = counts[2];
-  for (int k = gw; k < nch; k += waves) {
-    const Entry en = chunks[k];
-    Pair acc{0.0f, 0.0f};
-    for (int i = en.beg + lane; i < en.end; i += kWave) two_sum_add(acc, contrib_cur[nlist[i]]);
-    acc = wave_merge(acc);
-    if (lane == 0) partial[k] = acc;
-  }
-  round_partial_d(ctl, 0, err, s_red);
+  round_partial_ll(ctl, 0, err, s_red);
 }
 
 __global__ void __launch_bounds__(kBlock) pr_bal_big(const int* __restrict__ nindex, float base,
                                                       float* __restrict__ score, float* __restrict__ contrib_next,
                                                       const BigRow* __restrict__ big, const int* __restrict__ counts,
-                                                      const Pair* __restrict__ partial, Ctl* ctl, double eps,
-                                                      int max_iter)
+                                                      const Pair* __restrict__ partial, Ctl* ctl)
 {
-  __shared__ double s_red[kWavesPerBlock];
+  __shared__ long long s_red[kWavesPerBlock];
   if (round_done(ctl)) return;
-  double err = 0.0;
+  long long err = 0;
   const int nb = counts[1];
   const int stride = gridDim.x * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nb; i += stride) {
@@ -300,7 +313,7 @@ __global__ void __launch_bounds__(kBlock) pr_bal_big(const int* __restrict__ nin
     for (int k = 0; k < b.count; k++) acc = pair_merge(acc, partial[b.first + k]);
     err += pr_finish(b.v, acc, base, nindex, score, contrib_next);
   }
-  round_partial_d(ctl, 1, err, s_red);
+  round_partial_ll(ctl, 1, err, s_red);
 }
 
 // ------------------------------------------------------------ host side ----
@@ -334,7 +347,7 @@ int style_pull(const igx_graph* g, int, void* out, const void*, igx_run_info* in
         float* nxt = (r & 1) ? c0 : c1;
         hipLaunchKernelGGL((pr_pull<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                            g->nodes, base, score, cur, nxt, R.d_ctl, P.eps, P.max_iter);
-        launch_finalize(R.persist_blocks, 1 << kStats, FinPr{P.eps, P.max_iter});
+        launch_finalize(R.persist_blocks, 0x1, FinPr{P.eps, P.max_iter});
       },
       P.max_iter, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));
@@ -351,13 +364,14 @@ int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info
   float *c0 = nullptr, *c1 = nullptr;
   IGX_TRY(pr_buffers(g, &c0, &c1));
   const int n = g->nodes;
-  const size_t nbig = (size_t)g->edges / kMediumRow + 64;
-  const size_t nchunks = (size_t)g->edges / kChunk + nbig;
-  void *pmed = nullptr, *pcnt = nullptr, *pch = nullptr, *pbig = nullptr, *ppart = nullptr;
-  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)n, &pmed));
+  // rows above kSmallRow: at most edges / (kSmallRow + 1) of them, each with
+  // at most deg / kChunk + 1 chunks
+  const size_t nrows = (size_t)g->edges / (kSmallRow + 1) + 64;
+  const size_t nchunks = (size_t)g->edges / kChunk + nrows;
+  void *pcnt = nullptr, *pch = nullptr, *pbig = nullptr, *ppart = nullptr;
   IGX_TRY(ws_get(WS_B, sizeof(int) * 64, &pcnt));
   IGX_TRY(ws_get(WS_D, sizeof(Entry) * nchunks, &pch));
-  IGX_TRY(ws_get(WS_C, sizeof(BigRow) * nbig, &pbig));
+  IGX_TRY(ws_get(WS_C, sizeof(BigRow) * nrows, &pbig));
   IGX_TRY(ws_get(WS_H, sizeof(Pair) * nchunks, &ppart));
   const PrParams P = pr_params();
   const float base = (1.0f - kDamp) / (float)n;
@@ -365,8 +379,8 @@ int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info
   IGX_TRY(ctl_reset(n));
   hipLaunchKernelGGL(pr_zero, dim3(1), dim3(64), 0, R.stream, (int*)pcnt, 64);
   hipLaunchKernelGGL(pr_init, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, R.stream, g->nindex, n, score, c0);
-  hipLaunchKernelGGL(pr_classify, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, n, (int*)pmed,
-                     (int*)pcnt, (Entry*)pch, (BigRow*)pbig);
+  hipLaunchKernelGGL(pr_classify, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, n, (int*)pcnt,
+                     (Entry*)pch, (BigRow*)pbig);
   int rounds = 0, launches = 0;
   const int rc = run_rounds(
       [&](long long r) {
@@ -374,13 +388,11 @@ int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info
         float* nxt = (r & 1) ? c0 : c1;
         ktimer_begin("pr_bal_main");
         hipLaunchKernelGGL(pr_bal_main, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n,
-                           base, score, cur, nxt, (const int*)pmed, (const int*)pcnt, (const Entry*)pch,
-                           (Pair*)ppart, R.d_ctl);
+                           base, score, cur, nxt, (const int*)pcnt, (const Entry*)pch, (Pair*)ppart, R.d_ctl);
         ktimer_end("pr_bal_main");
         hipLaunchKernelGGL(pr_bal_big, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, base, score,
-                           nxt, (const BigRow*)pbig, (const int*)pcnt, (const Pair*)ppart, R.d_ctl, P.eps,
-                           P.max_iter);
-        launch_finalize(R.persist_blocks, 3 << kStats, FinPr{P.eps, P.max_iter});
+                           nxt, (const BigRow*)pbig, (const int*)pcnt, (const Pair*)ppart, R.d_ctl);
+        launch_finalize(R.persist_blocks, 0x3, FinPr{P.eps, P.max_iter});
       },
       P.max_iter, &rounds, &launches);
   IGX_TRY(timer_stop(&info->seconds));

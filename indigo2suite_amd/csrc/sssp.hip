@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
                                                          int n, long long m, int* __restrict__ dist, SegQ q0,
                                                          SegQ q1, const Entry* __restrict__ bigq,
                                                          const int* __restrict__ bigq_count,
-                                                         unsigned* __restrict__ bm, int words, Ctl* ctl)
+                                                         unsigned* __restrict__ bm, int words, int grab, Ctl* ctl)
 {
   __shared__ int s_excl[kWavesPerBlock][kWave];
   __shared__ int s_cand[kWavesPerBlock][kSlice];
@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     // edges: the wave loads kSlice / 64 edges per lane with independent loads
     // (ids, frontier bits, distances, weights), parks the candidates in LDS,
     // and each lane then folds its own row's part of the slice.
-    // The tiles are claimed dynamically, two at a time: the blocks of group g
+    // The tiles are claimed dynamically, `grab` at a time: the blocks of group g
     // take the tiles g, g + kGroups, g + 2 kGroups, ... in turn (tile weights
     // vary by more than 4x on R-MAT and drift with the vertex id; the long
     // rows' chunks above by little), so waves finish together.
@@ -560,13 +560,12 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     const int ntiles = (n + kWave - 1) / kWave;
     const int grp = my_group();
     int* t_next = &ctl_groups(ctl)[grp].tiles;
-    constexpr int kGrab = 2;
     for (;;) {  // uniform per wave
       int k0 = 0;
-      if (lane == 0) k0 = atomicAdd(t_next, kGrab);
+      if (lane == 0) k0 = atomicAdd(t_next, grab);
       k0 = wave_bcast(k0, 0);
       if (grp + (long long)k0 * kGroups >= ntiles) break;
-      for (int kc = k0; kc < k0 + kGrab; kc++) {
+      for (int kc = k0; kc < k0 + grab; kc++) {
         const long long tl = grp + (long long)kc * kGroups;
         if (tl >= ntiles) break;  // uniform
         const int v0 = (int)tl * kWave;
@@ -816,6 +815,7 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
   int* dist = (int*)out;
   const long long cap = (long long)g->nodes + g->edges / kChunk + 64;  // chunks of all rows
   const int words = ((g->nodes + kWave - 1) / kWave) * 2;
+  const int grab = tile_grab(g);
   void *p0 = nullptr, *p1 = nullptr, *pm = nullptr, *pb = nullptr, *pc = nullptr;
   IGX_TRY(ws_get(WS_D, sizeof(Entry) * (size_t)cap * kGroups, &p0));
   IGX_TRY(ws_get(WS_E, sizeof(Entry) * (size_t)cap * kGroups, &p1));
@@ -836,7 +836,7 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
         ktimer_begin("sssp_do_round");
         hipLaunchKernelGGL(sssp_do_round, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                            g->eweight, g->nodes, (long long)g->edges, dist, q0, q1, (const Entry*)pb,
-                           (const int*)pc, (unsigned*)pm, words, R.d_ctl);
+                           (const int*)pc, (unsigned*)pm, words, grab, R.d_ctl);
         ktimer_end("sssp_do_round");
         launch_finalize(R.persist_blocks, 0xF | (1 << kStatMin), FinSsspDo{(long long)g->edges});
       },
