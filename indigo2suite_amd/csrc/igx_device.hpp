@@ -307,57 +307,77 @@ struct RoundTotals {
   double d[kStatsD];
 };
 
-constexpr int kFinBlock = 256;
+constexpr int kFinBlock = 1024;  // 16 waves: two partials per thread for a 2048-block round
 
 // Finalize kernel (one workgroup of kFinBlock threads): reduces the partials
 // of the nparts blocks of the round kernel for the statistics in `stat_mask`
 // (bit k: integer k, summed, or the minimum for kStatMin; bit kStats + j:
-// FP64 j, summed), then runs fin(ctl, totals) on one thread.  Does nothing
-// once the run is done.
+// FP64 j, summed), then runs fin(ctl, totals) on one thread.  All statistics
+// are reduced in one pass: every thread loads its strided partials of each
+// (independent loads), the waves reduce by shuffles and thread 0 adds the
+// waves in order, so FP64 totals are the same on every run.  Does nothing once
+// the run is done.
 template <typename Fin>
 __global__ void __launch_bounds__(kFinBlock) round_finalize(Ctl* ctl, int nparts, int stat_mask, Fin fin)
 {
-  __shared__ long long s_l[kFinBlock];
-  __shared__ double s_d[kFinBlock];
-  __shared__ RoundTotals tot;
+  constexpr int kFinWaves = kFinBlock / kWave;
+  __shared__ long long s_l[kFinWaves][kStats];
+  __shared__ double s_d[kFinWaves][kStatsD];
   if (round_done(ctl)) return;
-  for (int k = 0; k < kStats + kStatsD; k++) {
-    if (!((stat_mask >> k) & 1)) {
-      if (threadIdx.x == 0) {
-        if (k < kStats) tot.s[k] = 0;
-        else tot.d[k - kStats] = 0.0;
+  long long acc[kStats];
+  double accd[kStatsD];
+  #pragma unroll
+  for (int k = 0; k < kStats; k++) acc[k] = (k == kStatMin) ? LLONG_MAX : 0;
+  #pragma unroll
+  for (int k = 0; k < kStatsD; k++) accd[k] = 0.0;
+  const long long* pl = ctl_parts(ctl);
+  const double* pd = ctl_parts_d(ctl);
+  for (int b0 = threadIdx.x; b0 < nparts; b0 += 2 * kFinBlock) {
+    const int b1 = b0 + kFinBlock;  // two partials per statistic, loaded together
+    #pragma unroll
+    for (int k = 0; k < kStats; k++) {
+      if ((stat_mask >> k) & 1) {
+        const bool is_min = k == kStatMin;
+        const long long x0 = pl[(size_t)k * kMaxParts + b0];
+        const long long x1 = (b1 < nparts) ? pl[(size_t)k * kMaxParts + b1] : (is_min ? LLONG_MAX : 0);
+        acc[k] = is_min ? min(acc[k], min(x0, x1)) : acc[k] + x0 + x1;
       }
-      continue;  // uniform
     }
-    const bool is_min = k == kStatMin;
-    if (k < kStats) {
-      const long long* p = ctl_parts(ctl) + (size_t)k * kMaxParts;
-      long long t = is_min ? LLONG_MAX : 0;
-      for (int b = threadIdx.x; b < nparts; b += kFinBlock) t = is_min ? min(t, p[b]) : t + p[b];
-      s_l[threadIdx.x] = t;
-    } else {
-      const double* p = ctl_parts_d(ctl) + (size_t)(k - kStats) * kMaxParts;
-      double t = 0.0;
-      for (int b = threadIdx.x; b < nparts; b += kFinBlock) t += p[b];
-      s_d[threadIdx.x] = t;
-    }
-    __syncthreads();
-    for (int off = kFinBlock / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) {
-        if (is_min) s_l[threadIdx.x] = min(s_l[threadIdx.x], s_l[threadIdx.x + off]);
-        else if (k < kStats) s_l[threadIdx.x] += s_l[threadIdx.x + off];
-        else s_d[threadIdx.x] += s_d[threadIdx.x + off];
+    #pragma unroll
+    for (int k = 0; k < kStatsD; k++) {
+      if ((stat_mask >> (kStats + k)) & 1) {
+        const double x0 = pd[(size_t)k * kMaxParts + b0];
+        const double x1 = (b1 < nparts) ? pd[(size_t)k * kMaxParts + b1] : 0.0;
+        accd[k] += x0 + x1;
       }
-      __syncthreads();
     }
-    if (threadIdx.x == 0) {
-      if (k < kStats) tot.s[k] = s_l[0];
-      else tot.d[k - kStats] = s_d[0];
-    }
-    __syncthreads();
+  }
+  const int w = threadIdx.x / kWave;
+  #pragma unroll
+  for (int k = 0; k < kStats; k++) {
+    const long long x = (k == kStatMin) ? wave_min_ll(acc[k]) : wave_sum_ll(acc[k]);
+    if (lane_id() == 0) s_l[w][k] = x;
+  }
+  #pragma unroll
+  for (int k = 0; k < kStatsD; k++) {
+    const double x = wave_sum_d(accd[k]);
+    if (lane_id() == 0) s_d[w][k] = x;
   }
   __syncthreads();
-  if (threadIdx.x == 0) fin(ctl, tot);
+  if (threadIdx.x == 0) {
+    RoundTotals tot;
+    for (int k = 0; k < kStats; k++) {
+      long long t = (k == kStatMin) ? LLONG_MAX : 0;
+      for (int i = 0; i < kFinWaves; i++) t = (k == kStatMin) ? min(t, s_l[i][k]) : t + s_l[i][k];
+      tot.s[k] = ((stat_mask >> k) & 1) ? t : 0;
+    }
+    for (int k = 0; k < kStatsD; k++) {
+      double t = 0.0;
+      for (int i = 0; i < kFinWaves; i++) t += s_d[i][k];
+      tot.d[k] = ((stat_mask >> (kStats + k)) & 1) ? t : 0.0;
+    }
+    fin(ctl, tot);
+  }
 }
 
 
