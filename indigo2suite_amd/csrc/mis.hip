@@ -220,6 +220,11 @@ __global__ void __launch_bounds__(kBlock) mis_vertex_pass(int n, int* __restrict
 // excluded (in the set -> v is excluded; undecided -> v waits).  Where it
 // stopped is kept in pos[v] (every outranking neighbour before it is excluded,
 // which is final), so later rounds resume there instead of rescanning.
+// Statuses are read and written with plain loads and stores, not agent-scope
+// atomics (on gfx950 those bypass the XCD's L2 and made the first round 8x
+// slower): a status only ever moves from undecided to in or out, so a stale
+// read can only say "undecided", and then v waits a round; every decision is
+// taken on a value that was true.  The kernel boundary publishes the round.
 // A lane walks at most kMisLane neighbours on its own per round and pushes
 // rows of at most kMisLane edges itself; longer walks and pushes are finished
 // by the whole wave, 64 neighbours per step.  The fixed point is the same
@@ -244,7 +249,7 @@ __global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict_
   bool pending = false;
   for (int base = gw * kWave; base < n; base += nwaves * kWave) {  // uniform per wave
     const int v = base + lane;
-    const bool undec = (v < n) && load_relaxed(&status[v]) == kUndecided;
+    const bool undec = (v < n) && status[v] == kUndecided;
     const uint32_t pv = mis_priority((uint32_t)v);
     int i = 0, end = 0, result = kUndecided;
     bool more = false;
@@ -255,7 +260,7 @@ __global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict_
       for (; i < end && steps < kMisLane; i++, steps++) {
         const int u = nlist[i];
         if (u == v || !outranks(mis_priority((uint32_t)u), u, pv, v)) continue;
-        const int su = load_relaxed(&status[u]);
+        const int su = status[u];
         if (su == kIncluded) {
           result = kExcluded;
           break;
@@ -278,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict_
         int su = kExcluded;
         if (idx < e0) {
           const int u = nlist[idx];
-          if (u != vL && outranks(mis_priority((uint32_t)u), u, pvL, vL)) su = load_relaxed(&status[u]);
+          if (u != vL && outranks(mis_priority((uint32_t)u), u, pvL, vL)) su = status[u];
         }
         const uint64_t hit = ballot(su != kExcluded);
         if (hit != 0) {
@@ -297,7 +302,7 @@ __global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict_
     int rb = 0;
     if (undec) {
       if (result != kUndecided) {
-        store_relaxed(&status[v], result);
+        status[v] = result;
         rb = nindex[v];
       } else {
         pos[v] = i;
@@ -310,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict_
     if (short_row) {
       for (int j = rb; j < end; j++) {
         const int u = nlist[j];
-        if (u != v) store_relaxed(&status[u], kExcluded);
+        if (u != v) status[u] = kExcluded;
       }
     }
     uint64_t big = ballot(joined && !short_row);
@@ -321,7 +326,7 @@ __global__ void __launch_bounds__(kBlock) mis_resume_pull(const int* __restrict_
       const int s0 = __shfl(rb, L, kWave), e0 = __shfl(end, L, kWave);
       for (int j = s0 + lane; j < e0; j += kWave) {
         const int u = nlist[j];
-        if (u != vL) store_relaxed(&status[u], kExcluded);
+        if (u != vL) status[u] = kExcluded;
       }
     }
   }
