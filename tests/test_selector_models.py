@@ -1,7 +1,8 @@
 """Shipped selector models and timing datasets: their style lists match the
 library's registry (CPU), and on the held-out graphs of 2^19 to 2^20.5
-vertices the trained picks stay close to the exhaustive best and are never
-slower than the default style on average (GPU: the MLP runs on the device)."""
+vertices the trained picks stay within a few percent of the exhaustive best
+and no slower than the default style on average (GPU: the MLP runs on the
+device)."""
 import os
 
 import numpy as np
@@ -41,9 +42,9 @@ def test_shipped_models_on_held_out_graphs(gpu):
         try:
             dflt = D.default_entry_style(algo)
             ev = sel.evaluate(d["features"], d[f"times_{algo}"], dflt)
-            assert ev["geomean_speedup_vs_default"] >= 0.99, (algo, ev)
-            assert ev["frac_within_10pct"] >= 0.8, (algo, ev)
-            assert ev["mean_regret"] <= (0.2 if algo == "cc" else 0.1), (algo, ev)
+            assert ev["geomean_speedup_vs_default"] >= 0.98, (algo, ev)
+            assert ev["frac_within_10pct"] >= 0.9, (algo, ev)
+            assert ev["mean_regret"] <= 0.05, (algo, ev)
         finally:
             sel.net.free()
 
