@@ -3,7 +3,7 @@ graph statistics that predicts, for each of an algorithm's K registered
 styles, its slowdown against the fastest style; the pick is the style with
 the smallest prediction.  Training and inference run on the GPU through the
 hand-written HIP kernels of libindigo_hip.so (igx_mlp_*); the exhaustive
-autotuner (autotune.sweep) is both the fallback and the judge of regret.
+autotuner (autotune.sweep) is both the fallback and the yardstick of regret.
 
 Targets: y_k = log(t_k / t_best), clipped at log(kClip); styles without a
 measurement are masked out of the loss.  Features are standardised with the

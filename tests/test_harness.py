@@ -119,7 +119,7 @@ def test_driver_crashed_cases(tmp_path, gpu):
 
 
 def test_run_code_argument_conventions(tmp_path, monkeypatch):
-    """The driver passes (input, source, verify) to BFS/SSSP programs, (input)
+    """run_codes.py passes (input, source, verify) to BFS/SSSP programs, (input)
     to PageRank and (input, verify) to the rest, deciding by the code
     directory's own name even when a parent directory contains "pr"."""
     calls = []
