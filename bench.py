@@ -14,14 +14,23 @@ formula, bfs_vertex_cuda.h:163, applied to the two runs of a step).
 
 roofline: the SSSP round kernel (sssp_do_round), the step's dominant kernel,
 timed with HIP events on the library stream over the timed steps.  Its
-algorithmic bytes are counted on the device: 4 bytes per edge scanned
-(neighbour id), 8 more per edge relaxed (weight, neighbour distance) and 12
-per row or queue entry visited (row offsets, distance); a pull round that
-relaxes every edge is 12m + 12n, the SSSP figure per run.
+algorithmic bytes are the style-independent SSSP figure, 12 bytes per
+directed edge (neighbour id, weight, neighbour distance) and 12 per vertex
+(row offsets, distance read and write): 12m + 12n + 4 per run, spread over
+the run's launches.  achieved = those bytes / the summed launch time.  The
+bytes the kernel's executed work touches (4 per edge scanned, 8 more per
+edge relaxed, 12 per row visited, counted on the device) are reported beside
+it, as is the copy-calibrated HBM traffic per launch from rocprofv3.
 
 cpu_baseline: the oracle's OpenMP restatement of the reference's fastest
-OpenMP styles (SSSP and CC vertex-based topology-driven pull) on a bounded
-sample of the same workload: the same generator at scale 20.
+OpenMP styles (SSSP and CC vertex-based topology-driven pull) on the same
+R-MAT 22 graph.  Beside it, the style-matched restatements (SSSP edge-based
+data-driven push and CC vertex-based topology-driven push, both serialised by
+the reference's global critical section) on a bounded sample: the same
+generator at scale 16.
+
+selector: on the bench graph, for SSSP and CC, the learned selector's pick,
+the library default and the exhaustive sweep's best style, each timed.
 
 usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -71,8 +80,9 @@ def verify_sssp(g, src, dist):
     w = g.eweight.astype(np.int64)
     assert dist[src] == 0
     ok = du != INF
+    # every edge out of a reached vertex is relaxed; the graph is symmetric,
+    # so this also rules out a reached neighbour of an unreached vertex
     assert np.all(dv[ok] <= du[ok] + w[ok])
-    assert np.all(dv[~ok] == INF) or True  # undirected: unreached neighbours of unreached
     tight = np.zeros(g.nodes, bool)
     tight[g.nlist[ok & (dv == du + w)]] = True
     assert np.all(tight[(d != INF) & (np.arange(g.nodes) != src)])
@@ -155,6 +165,52 @@ def measured_traffic(scale):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_style_matched(scale=16, reps=3):
+    """The oracle's restatements of the style-matched reference OpenMP
+    variants ([O-SSSP] edge data-driven push, [O-CC] vertex topology push) on
+    R-MAT `scale`: median of `reps` runs each."""
+    import oracle as O
+    from indigo2suite_amd import graph as G
+    g = G.rmat(scale, 16, seed=1).with_weights(1, 255, seed=1)
+    src = g.max_degree_vertex()
+    ts, tc = [], []
+    for _ in range(reps):
+        ts.append(O.sssp_omp_edge_data(g, src)[1])
+        tc.append(O.cc_omp_push(g)[1])
+    t1, t2 = float(np.median(ts)), float(np.median(tc))
+    return {"value": round(2 * g.edges / (t1 + t2) * 1e-9, 5), "unit": "GTEPS",
+            "sample": (f"R-MAT scale {scale} (n={g.nodes}, m={g.edges}): SSSP_OMP_E_Data_Push restatement "
+                       f"{t1:.3f} s + CC_OMP_V_Topo_Push restatement {t2:.3f} s, median of {reps}")}
+
+
+def selector_section(D, dg, g, src, algos=("sssp", "cc")):
+    """Learned pick vs the library default vs the exhaustive sweep's best, on
+    the bench graph, for the step's algorithms."""
+    from indigo2suite_amd import autotune, selector
+    out = {}
+    dmax = int(g.degrees().max())
+    for a in algos:
+        pick = selector.pick_style(g, a)
+        default = D.styles(a)[D.default_entry_style(a, 0)]
+        sw = autotune.sweep(dg, a, src, repeats=3, max_degree=dmax)
+        best = min(sw, key=sw.get)
+        out[a] = {"pick": pick, "pick_s": round(sw[pick], 6), "default": default, "default_s": round(sw[default], 6),
+                  "best": best, "best_s": round(sw[best], 6),
+                  "regret_vs_best": round(sw[pick] / sw[best] - 1.0, 4),
+                  "speedup_vs_default": round(sw[default] / sw[pick], 4), "styles_swept": len(sw)}
+    return out
+
+
 def cpu_baseline(scale, g22=None, src22=None, reps=5):
     """The oracle's OpenMP restatement of the fastest reference OpenMP styles
     (SSSP and CC vertex-based topology-driven pull) on the same workload,
@@ -175,6 +231,7 @@ def cpu_baseline(scale, g22=None, src22=None, reps=5):
     t1, t2 = float(np.median(ts)), float(np.median(tc))
     value = 2 * g.edges / (t1 + t2) * 1e-9
     return {"value": round(value, 4), "unit": "GTEPS", "cores": O.num_threads(), "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": (f"R-MAT scale {scale} (n={g.nodes}, m={g.edges}, same generator, seed 1): oracle OpenMP "
                        f"restatement of SSSP_OMP_V_Topo_Pull ({t1:.3f} s, {its} iterations) + "
                        f"CC_OMP_V_Topo_Pull ({t2:.3f} s, {itc} iterations), median of {reps} runs each")}
@@ -189,6 +246,7 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=22)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-selector", action="store_true", help="skip the learned-pick vs default vs best section")
     ap.add_argument("--no-extra", action="store_true", help="skip PageRank and TC on R-MAT 20 and the grid")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the known-size kernel that scales "
@@ -290,18 +348,25 @@ def main():
                                            "gteps": round(m / named.seconds * 1e-9, 3),
                                            "iterations": named.iterations}
 
-    # algorithmic bytes of the SSSP round kernel, counted on the device: 4 B
-    # per edge scanned, 8 B more per edge relaxed, 12 B per row or queue entry,
-    # summed over the timed launches (4 B each for the ones that found the run
-    # converged)
+    # roofline of the SSSP round kernel: the style-independent algorithmic
+    # bytes of the timed runs (12m + 12n + 4 each) over the summed duration of
+    # its launches in those runs
     launches = max(hook_n.value, 1)
     total_s = hook_ms.value * 1e-3
-    bytes_total = 4 * work_e + 8 * work_r + 12 * work_v + 4 * launches
-    achieved = bytes_total / total_s * 1e-9 if total_s > 0 else 0.0
+    alg_total = algorithmic_bytes("sssp", n, m) * args.steps
+    achieved = alg_total / total_s * 1e-9 if total_s > 0 else 0.0
+    # bytes of the work the launches executed, counted on the device
+    exec_total = 4 * work_e + 8 * work_r + 12 * work_v + 4 * launches
+    traffic = measured_traffic(args.scale)
+    alg_per_launch = alg_total / launches
     roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.scale),
-                "bytes_per_launch": int(bytes_total / launches), "avg_launch_us": round(total_s / launches * 1e6, 2),
-                "launches": hook_n.value, "edges_scanned_per_run": int(work_e / args.steps),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": int(alg_per_launch), "avg_launch_us": round(total_s / launches * 1e6, 2),
+                "launches": hook_n.value, "launches_per_run": round(hook_n.value / max(args.steps, 1), 2),
+                "traffic_over_algorithmic": round(traffic / alg_per_launch, 3) if traffic else None,
+                "executed_bytes_per_launch": int(exec_total / launches),
+                "executed_achieved": round(exec_total / total_s * 1e-9, 1) if total_s > 0 else 0.0,
+                "edges_scanned_per_run": int(work_e / args.steps),
                 "edges_relaxed_per_run": int(work_r / args.steps), "rows_visited_per_run": int(work_v / args.steps)}
 
     if args.copy_probe > 0:
@@ -312,10 +377,17 @@ def main():
         b.free()
         log(f"copy probe: {nb} bytes device to device")
 
+    sel = None
+    if not args.no_selector:
+        t = time.time()
+        sel = selector_section(D, dg, g, src)
+        log(f"selector section in {time.time() - t:.1f}s")
+
     cpu = None
     if not args.no_cpu:
         t = time.time()
         cpu = cpu_baseline(args.cpu_scale, g, src)
+        cpu["style_matched"] = cpu_style_matched()
         log(f"cpu baseline in {time.time() - t:.1f}s")
 
     line = {
@@ -328,6 +400,7 @@ def main():
                    "n": n, "m_directed": m, "parallelism": "single GPU"},
         "per_algo": per_algo,
         "roofline": roofline,
+        "selector": sel,
         "cpu_baseline": cpu,
         "device": dinfo["name"],
     }

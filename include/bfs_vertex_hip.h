@@ -20,10 +20,12 @@ int main(int argc, char* argv[])
   printf("style: %s\n", style_name(IGX_BFS, 0));
   std::vector<int> distance(g.nodes);
   igx_graph d_g = upload(g);
-  const int runs = 3;
-  double runtimes[runs];
-  for (int i = 0; i < runs; i++) check(igx_bfs_vertex(source, &d_g, distance.data(), &runtimes[i]), "igx_bfs_vertex");
-  report("", runtimes, runs, g.edges);
+  timed_runs_with_cutoff([&] {
+    double t = 0;
+    check(igx_bfs_vertex(source, &d_g, distance.data(), &t), "igx_bfs_vertex");
+    print_iterations();
+    return t;
+  }, g.edges);
   int maxnode = 0;
   for (int v = 1; v < g.nodes; v++) if (distance[maxnode] < distance[v]) maxnode = v;
   printf("vertex %d has maximum distance %d\n", maxnode, distance[maxnode]);

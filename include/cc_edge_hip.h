@@ -20,10 +20,11 @@ int main(int argc, char* argv[])
   igx_graph d_g = upload(g);
   const int runs = 3;
   double runtimes[runs];
-  for (int i = 0; i < runs; i++) check(igx_cc_edge(&d_g, label.data(), sp.data(), &runtimes[i]), "igx_cc_edge");
-  const double med = median(runtimes, runs);
-  printf("runtime: %.6fs\n", med);
-  printf("GPU Throughput: %.6f gigaedges/s\n", 0.000000001 * g.edges / med);
+  for (int i = 0; i < runs; i++) {
+    check(igx_cc_edge(&d_g, label.data(), sp.data(), &runtimes[i]), "igx_cc_edge");
+    print_iterations();
+  }
+  report(kCc, runtimes, runs, g.edges);
   std::set<int> s1(label.begin(), label.end());
   printf("number of connected components: %d\n", (int)s1.size());
   if (runveri) verify_components(g, label.data());

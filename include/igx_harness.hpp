@@ -59,7 +59,7 @@ inline int gpu_info(int d)
   char name[256];
   int cus = 0, mt = 0;
   check(igx_device_info(name, sizeof(name), &cus, &mt), "igx_device_info");
-  printf("GPU: %s with %d CUs and %d max threads per CU\n", name, cus, mt);
+  printf("GPU: %s with %d CUs and %d mTpCU\n", name, cus, mt);
   return cus * mt;
 }
 
@@ -137,11 +137,56 @@ inline void serial_dijkstra(int src, const igx_graph& g, bool weighted, std::vec
   }
 }
 
-inline void report(const char* prefix, double* runtimes, int runs, int edges)
+// The reference's result lines differ by harness; each harness header names
+// its form:
+//   kPlain      "runtime: %.6fs" / "Throughput: %.6f gigaedges/s"  (bfs_*_cuda.h:162-163, mis_vertex_cuda.h:160-161)
+//   kGpuSpaced  "GPU runtime: %.6f s" / "GPU Throughput: ..."      (sssp_*_cuda.h:166-167, tc_*_cuda.h:185-186)
+//   kCc         "runtime: %.6fs" / "GPU Throughput: ..."           (cc_*_cuda.h:131-132)
+//   kMisEdge    "runtime: %.6fs" / "Throughput: %.6f gigabytes/s"  (mis_edge_cuda.h:170-171)
+enum ReportForm { kPlain, kGpuSpaced, kCc, kMisEdge };
+
+inline void report_time(ReportForm form, double t, int edges)
 {
-  const double med = median(runtimes, runs);
-  printf("%sruntime: %.6fs\n", prefix, med);
-  printf("%sThroughput: %.6f gigaedges/s\n", prefix, 0.000000001 * edges / med);
+  const double gteps = 0.000000001 * edges / t;
+  switch (form) {
+    case kPlain: printf("runtime: %.6fs\nThroughput: %.6f gigaedges/s\n", t, gteps); break;
+    case kGpuSpaced: printf("GPU runtime: %.6f s\nGPU Throughput: %.6f gigaedges/s\n", t, gteps); break;
+    case kCc: printf("runtime: %.6fs\nGPU Throughput: %.6f gigaedges/s\n", t, gteps); break;
+    case kMisEdge: printf("runtime: %.6fs\nThroughput: %.6f gigabytes/s\n", t, gteps); break;
+  }
+}
+
+inline void report(ReportForm form, double* runtimes, int runs, int edges)
+{
+  report_time(form, median(runtimes, runs), edges);
+}
+
+// iteration count of the entry call just made, printed as the reference's
+// variants print it inside their GPU* function (e.g. [C-BVT]:111)
+inline int last_iterations()
+{
+  igx_run_info info = {};
+  check(igx_last_run_info(&info), "igx_last_run_info");
+  return info.iterations;
+}
+
+inline void print_iterations() { printf("iterations: %d\n", last_iterations()); }
+
+// BFS harness timing loop (bfs_vertex_cuda.h:147-164): three runs and their
+// median, except that a run of 30 s or more is reported alone and ends the loop
+template <typename Run>
+inline void timed_runs_with_cutoff(Run run, int edges)
+{
+  const int runs = 3;
+  double runtimes[runs];
+  for (int i = 0; i < runs; i++) {
+    runtimes[i] = run();
+    if (runtimes[i] >= 30.0) {
+      report_time(kPlain, runtimes[i], edges);
+      return;
+    }
+  }
+  report(kPlain, runtimes, runs, edges);
 }
 
 // verify BFS / SSSP distances exactly (bfs_vertex_cuda.h:174-184)

@@ -98,6 +98,13 @@ typedef struct igx_run_info {
 int igx_run(int algo, int style, const igx_graph* dev, int src, void* d_out, const int* d_aux_sp,
             igx_run_info* info);
 
+/* Run info (iterations, launches, seconds) of the last reference-signature
+ * entry call below: what the reference's variant prints inside its GPU*
+ * function ("iterations: %d", e.g. [C-BVT]:111; MIS "iterations: %d,
+ * elements in set: ...", [C-MVT]:122), which the entries' signatures do not
+ * return. */
+int igx_last_run_info(igx_run_info* info);
+
 /* Per-kernel timing (for roofline measurements).  After
  * igx_profile_kernel("uf_hook") every launch of that kernel is bracketed by
  * events on the library stream; igx_profile_read() waits for the stream and

@@ -29,11 +29,11 @@ int main(int argc, char* argv[])
     check(igx_mis_edge(&d_g, (const int*)d_sp, priority.data(), status.data(), &runtimes[i]), "igx_mis_edge");
     int cnt = 0;
     for (int v = 0; v < g.nodes; v++) cnt += (status[v] == 1);
-    printf("elements in set: %d (%.1f%%)\n", cnt, 100.0 * cnt / g.nodes);
+    printf("iterations: %d,  elements in set: %d (%.1f%%)\n", last_iterations(), cnt, 100.0 * cnt / g.nodes);
     if (runveri) verify_mis(g, status.data());
     else printf("turn off verification\n\n");
   }
-  report("", runtimes, runs, g.edges);
+  report(kMisEdge, runtimes, runs, g.edges);
   igx_free(d_sp);
   igx_graph_free_device(&d_g);
   igx_host_graph_free(&g);

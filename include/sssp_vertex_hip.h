@@ -23,8 +23,11 @@ int main(int argc, char* argv[])
   igx_graph d_g = upload(g);
   const int runs = 3;
   double runtimes[runs];
-  for (int i = 0; i < runs; i++) check(igx_sssp_vertex(source, &d_g, distance.data(), &runtimes[i]), "igx_sssp_vertex");
-  report("GPU ", runtimes, runs, g.edges);
+  for (int i = 0; i < runs; i++) {
+    check(igx_sssp_vertex(source, &d_g, distance.data(), &runtimes[i]), "igx_sssp_vertex");
+    print_iterations();
+  }
+  report(kGpuSpaced, runtimes, runs, g.edges);
   int maxnode = 0;
   for (int v = 1; v < g.nodes; v++) if (distance[maxnode] < distance[v]) maxnode = v;
   printf("vertex %d has maximum distance %d\n", maxnode, distance[maxnode]);

@@ -25,7 +25,7 @@ int main(int argc, char* argv[])
   const int runs = 3;
   double runtimes[runs];
   for (int i = 0; i < runs; i++) check(igx_tc_edge(&count, g.edges, d_g.nindex, d_g.nlist, (const int*)d_sp, &runtimes[i]), "igx_tc_edge");
-  report("GPU ", runtimes, runs, g.edges);
+  report(kGpuSpaced, runtimes, runs, g.edges);
   const int verify = atoi(argv[2]);
   if ((verify != 0) && (verify != 1)) printf("has to be 0 (turn off) or 1 (turn on) verification");
   if (verify) {

@@ -319,6 +319,11 @@ __global__ void batched_reset(int* counters, int count)
 }
 
 // ------------------------------------------------------------ host side ----
+// The batched mode owns its buffers (the three queue counters and the
+// per-vertex scratch) instead of borrowing the library's shared workspaces: a
+// captured graph keeps the addresses it was recorded with, and a shared slot
+// may be freed and regrown by any other entry point between two replays.
+// The buffers only ever change together with dropping the captured graph.
 struct BatchState {
   igx_batch key = {};
   int block = 0;
@@ -326,8 +331,16 @@ struct BatchState {
   int* outs[3] = {nullptr, nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int* counters = nullptr;     // kCounterInts ints
+  int* scratch = nullptr;      // scratch_ints ints
+  size_t scratch_ints = 0;
 };
 BatchState g_state;
+constexpr int kCounterInts = 64;
+
+// scratch ints of a batch: BFS levels queue pair (2 per vertex), MIS
+// worklist pair (2 per vertex), and a small tail
+inline size_t scratch_ints_for(const igx_batch* b) { return 4 * (size_t)b->total_nodes + 64; }
 
 template <int B>
 void launch_all(const igx_batch* b, const Arena& A, int mask, int* const* outs, int* counters, int* scratch, int grid,
@@ -341,26 +354,68 @@ void launch_all(const igx_batch* b, const Arena& A, int mask, int* const* outs, 
                                    scratch + 2 * b->total_nodes);
 }
 
-int enqueue(const igx_batch* b, int mask, int* const* outs, int block, hipStream_t s)
+void drop_graph()
+{
+  if (g_state.exec) hipGraphExecDestroy(g_state.exec);
+  if (g_state.graph) hipGraphDestroy(g_state.graph);
+  g_state.exec = nullptr;
+  g_state.graph = nullptr;
+}
+
+// Make the batch buffers hold at least `ints` scratch ints.  Growing them
+// drops the captured graph first (it refers to the old addresses).
+int batch_buffers(size_t ints)
+{
+  if (g_state.counters == nullptr) {
+    if (hipMalloc((void**)&g_state.counters, sizeof(int) * kCounterInts) != hipSuccess) {
+      (void)hipGetLastError();
+      g_state.counters = nullptr;
+      return IGX_ERR_NOMEM;
+    }
+  }
+  if (g_state.scratch_ints >= ints && g_state.scratch != nullptr) return IGX_OK;
+  drop_graph();
+  g_state.key = igx_batch{};
+  if (g_state.scratch != nullptr) {
+    IGX_HIP(hipStreamSynchronize(rt().stream));
+    IGX_HIP(hipFree(g_state.scratch));
+    g_state.scratch = nullptr;
+    g_state.scratch_ints = 0;
+  }
+  if (hipMalloc((void**)&g_state.scratch, sizeof(int) * ints) != hipSuccess) {
+    (void)hipGetLastError();
+    g_state.scratch = nullptr;
+    return IGX_ERR_NOMEM;
+  }
+  g_state.scratch_ints = ints;
+  return IGX_OK;
+}
+
+// Enqueue the batch's kernels on stream s with the given counter and scratch
+// buffers (kCounterInts and scratch_ints_for(b) ints).
+int enqueue_with(const igx_batch* b, int mask, int* const* outs, int block, int* counters, int* scratch,
+                 hipStream_t s)
 {
   Runtime& R = rt();
-  void* pc = nullptr;
-  void* ps = nullptr;
-  IGX_TRY(ws_get(WS_SCRATCH, sizeof(int) * 64, &pc));
-  IGX_TRY(ws_get(WS_H, sizeof(int) * (size_t)(4 * b->total_nodes + 64), &ps));
   Arena A{b->node_off, b->edge_off, b->nindex, b->nlist, b->graphs};
   const int per_cu = R.max_threads_per_cu / block;
   int grid = R.cus * (per_cu > 0 ? per_cu : 1);
   if (grid > b->graphs) grid = b->graphs;
   if (grid < 1) grid = 1;
   switch (block) {
-    case 256: launch_all<256>(b, A, mask, outs, (int*)pc, (int*)ps, grid, s); break;
-    case 512: launch_all<512>(b, A, mask, outs, (int*)pc, (int*)ps, grid, s); break;
-    case 1024: launch_all<1024>(b, A, mask, outs, (int*)pc, (int*)ps, grid, s); break;
+    case 256: launch_all<256>(b, A, mask, outs, counters, scratch, grid, s); break;
+    case 512: launch_all<512>(b, A, mask, outs, counters, scratch, grid, s); break;
+    case 1024: launch_all<1024>(b, A, mask, outs, counters, scratch, grid, s); break;
     default: return IGX_ERR_ARG;
   }
   IGX_HIP(hipGetLastError());
   return IGX_OK;
+}
+
+int enqueue(const igx_batch* b, int mask, int* const* outs, int block, hipStream_t s)
+{
+  IGX_TRY(batch_buffers(scratch_ints_for(b)));
+  return enqueue_with(b, mask, outs, block, g_state.counters, g_state.scratch, s);
 }
 
 bool same_batch(const igx_batch& a, const igx_batch& b)
@@ -370,15 +425,21 @@ bool same_batch(const igx_batch& a, const igx_batch& b)
          a.order == b.order;
 }
 
-void drop_graph()
+}  // namespace
+
+int batch_release()
 {
-  if (g_state.exec) hipGraphExecDestroy(g_state.exec);
-  if (g_state.graph) hipGraphDestroy(g_state.graph);
-  g_state.exec = nullptr;
-  g_state.graph = nullptr;
+  drop_graph();
+  g_state.key = igx_batch{};
+  if (g_state.counters != nullptr || g_state.scratch != nullptr) IGX_HIP(hipStreamSynchronize(rt().stream));
+  if (g_state.counters != nullptr) IGX_HIP(hipFree(g_state.counters));
+  if (g_state.scratch != nullptr) IGX_HIP(hipFree(g_state.scratch));
+  g_state.counters = nullptr;
+  g_state.scratch = nullptr;
+  g_state.scratch_ints = 0;
+  return IGX_OK;
 }
 
-}  // namespace
 }  // namespace igx
 
 using namespace igx;
@@ -485,10 +546,8 @@ int igx_batch_run(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int*
                       g_state.outs[2] == d_mis && g_state.exec != nullptr;
     if (!same) {
       drop_graph();
-      // workspaces must exist before capture (no allocation inside it)
-      void* tmp = nullptr;
-      IGX_TRY(ws_get(WS_SCRATCH, sizeof(int) * 64, &tmp));
-      IGX_TRY(ws_get(WS_H, sizeof(int) * (size_t)(4 * b->total_nodes + 64), &tmp));
+      // buffers must exist before capture (no allocation inside it)
+      IGX_TRY(batch_buffers(scratch_ints_for(b)));
       IGX_HIP(hipStreamSynchronize(R.stream));
       IGX_HIP(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
       const int rc = enqueue(b, algo_mask, outs, block, R.stream);

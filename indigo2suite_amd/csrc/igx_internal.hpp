@@ -150,8 +150,11 @@ struct PrParams {
 };
 PrParams& pr_params();
 
-// device sp[] of the graph, built on the device and cached per graph pointer
+// device sp[] of the graph, built on the device into the WS_SP workspace
 int device_sp(const igx_graph* g, const int** d_sp);
+
+// Free the batched mode's own buffers and its captured graph.
+int batch_release();
 
 // Enqueue the construction of the chunk list of every row longer than the
 // frontier chunk (igx_frontier.hpp): bigq receives the chunks, *d_count their

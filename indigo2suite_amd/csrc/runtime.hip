@@ -19,6 +19,7 @@ char g_last_error[256] = "";
 void* g_ws[WS_COUNT] = {};
 size_t g_ws_bytes[WS_COUNT] = {};
 PrParams g_pr;
+igx_run_info g_last_info = {};  // run info of the last reference-signature entry call
 int g_entry_style[IGX_NUM_ALGOS][2];
 bool g_entry_style_set = false;
 
@@ -27,10 +28,6 @@ char g_prof_tag[64] = "";
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 size_t g_prof_used = 0;
 bool g_prof_open = false;
-
-// cached device sp[] keyed by the nlist pointer and edge count
-const int* g_sp_key = nullptr;
-int g_sp_edges = -1;
 
 __global__ void ctl_reset_kernel(Ctl* ctl, int n)
 {
@@ -102,7 +99,6 @@ int ws_get(int slot, size_t bytes, void** ptr)
     }
     g_ws[slot] = p;
     g_ws_bytes[slot] = bytes;
-    if (slot == WS_SP) {g_sp_key = nullptr; g_sp_edges = -1;}
   }
   *ptr = g_ws[slot];
   return IGX_OK;
@@ -120,17 +116,13 @@ int ctl_reset(int n)
 
 int device_sp(const igx_graph* g, const int** d_sp)
 {
-  if (g_sp_key == g->nlist && g_sp_edges == g->edges && g_ws[WS_SP] != nullptr) {
-    *d_sp = (const int*)g_ws[WS_SP];
-    return IGX_OK;
-  }
+  // Rebuilt on every call: a cache keyed by device pointers goes stale when
+  // a caller frees a graph and uploads another one at the same address.
   void* p = nullptr;
   IGX_TRY(ws_get(WS_SP, sizeof(int) * (size_t)(g->edges > 0 ? g->edges : 1), &p));
   hipLaunchKernelGGL(build_sp_kernel, dim3(blocks_for((long long)g->nodes * kWave, kBlock)), dim3(kBlock), 0,
                      g_rt.stream, g->nindex, g->nodes, (int*)p);
   IGX_HIP(hipGetLastError());
-  g_sp_key = g->nlist;
-  g_sp_edges = g->edges;
   *d_sp = (const int*)p;
   return IGX_OK;
 }
@@ -487,12 +479,17 @@ int igx_release_workspace(void)
     g_ws[i] = nullptr;
     g_ws_bytes[i] = 0;
   }
-  g_sp_key = nullptr;
-  g_sp_edges = -1;
-  return IGX_OK;
+  return batch_release();
 }
 
 const char* igx_last_error(void) { return g_last_error; }
+
+int igx_last_run_info(igx_run_info* info)
+{
+  if (info == nullptr) return IGX_ERR_ARG;
+  *info = g_last_info;
+  return IGX_OK;
+}
 
 const char* igx_status_string(int status)
 {
@@ -602,7 +599,6 @@ int igx_graph_upload(const igx_graph* host, igx_graph* dev)
 int igx_graph_free_device(igx_graph* dev)
 {
   if (dev == nullptr) return IGX_ERR_ARG;
-  if (g_sp_key == dev->nlist) {g_sp_key = nullptr; g_sp_edges = -1;}
   igx_free(dev->nindex);
   igx_free(dev->nlist);
   igx_free(dev->eweight);
@@ -729,7 +725,9 @@ static int run_to_host(int algo, int edge_kind, const igx_graph* g, int src, voi
   void* d_out = nullptr;
   IGX_TRY(ws_get(WS_SCRATCH, out_bytes, &d_out));
   const int style = igx_get_entry_style(algo, edge_kind);
+  g_last_info = igx_run_info{};
   IGX_TRY(igx_run(algo, style, g, src, d_out, d_sp, info));
+  if (info) g_last_info = *info;
   if (h_out != nullptr && out_bytes > 0) IGX_TRY(igx_memcpy_d2h(h_out, d_out, out_bytes));
   return IGX_OK;
 }

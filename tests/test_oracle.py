@@ -175,6 +175,116 @@ def test_tc_brute_force_tiny():
         assert O.tc_serial(g) == brute == O.tc_omp_vertex(g)[0] == O.tc_omp_edge(g)[0]
 
 
+def _is_mis_py(g, status):
+    """Maximal independent set by definition: every vertex decided, no two
+    included neighbours, every excluded vertex has an included neighbour."""
+    for v in range(g.nodes):
+        nb = g.nlist[g.nindex[v]:g.nindex[v + 1]]
+        if status[v] not in (1, 2):
+            return False
+        if status[v] == 1 and np.any(status[nb] == 1):
+            return False
+        if status[v] == 2 and not np.any(status[nb] == 1):
+            return False
+    return True
+
+
+def test_mis_verify_error_codes():
+    """mis_vertex_omp.h:54-73: undecided -> 1, adjacent included -> 2,
+    excluded without an included neighbour (not maximal) -> 3."""
+    g = G.chung_lu(3000, 8, 2.3, seed=4)
+    st = O.mis_greedy(g)
+    assert O.mis_verify(g, st) == 0
+    undecided = st.copy()
+    undecided[g.nodes // 2] = 0
+    assert O.mis_verify(g, undecided) == 1
+    # an excluded vertex with an included neighbour, switched to included
+    deg = np.diff(g.nindex)
+    v = next(v for v in range(g.nodes) if st[v] == 2 and deg[v] > 0)
+    adjacent = st.copy()
+    adjacent[v] = 1
+    assert O.mis_verify(g, adjacent) == 2
+    # an included vertex with neighbours, switched to excluded: nothing covers it
+    w = next(w for w in range(g.nodes) if st[w] == 1 and deg[w] > 0)
+    not_maximal = st.copy()
+    not_maximal[w] = 2
+    assert O.mis_verify(g, not_maximal) == 3
+    # an isolated vertex must be included
+    iso = G.from_edges(3, [0], [1])
+    assert O.mis_verify(iso, np.array([1, 2, 1], np.uint8)) == 0
+    assert O.mis_verify(iso, np.array([1, 2, 2], np.uint8)) == 3
+
+
+def test_mis_verify_brute_force_tiny():
+    """On tiny graphs the verifier accepts exactly the maximal independent
+    sets among all 2^n decided assignments (and rejects every one with an
+    undecided vertex)."""
+    rng = np.random.default_rng(5)
+    for trial in range(6):
+        n = 7
+        e = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < 0.35]
+        g = G.from_edges(n, [a for a, _ in e], [b for _, b in e])
+        accepted = 0
+        for mask in range(1 << n):
+            st = np.array([1 if (mask >> k) & 1 else 2 for k in range(n)], np.uint8)
+            ok = O.mis_verify(g, st) == 0
+            assert ok == _is_mis_py(g, st), (trial, mask)
+            accepted += ok
+        assert accepted >= 1
+        st = O.mis_greedy(g)
+        assert _is_mis_py(g, st)
+        st[rng.integers(n)] = 0
+        assert O.mis_verify(g, st) == 1
+
+
+def _pr_check_py(g, scores):
+    """pr_omp.h:85-104 in float64: sum over v of |(1-d)/n + d * sum over
+    in-neighbours u of s_u / deg_u - s_v|."""
+    d = 0.85
+    n = g.nodes
+    deg = np.diff(g.nindex).astype(np.float64)
+    s = scores.astype(np.float64)
+    out = np.where(deg > 0, s / np.where(deg > 0, deg, 1), 0.0)
+    incoming = np.zeros(n)
+    np.add.at(incoming, g.nlist, np.repeat(out, np.diff(g.nindex)))
+    return float(np.sum(np.abs((1 - d) / n + d * incoming - s)))
+
+
+def test_pr_check_detects_perturbed_scores():
+    g = G.rmat(12, 16, seed=2)
+    s64, _, _ = O.pr_fp64(g, 1e-12, 1000)
+    good = s64.astype(np.float32)
+    assert O.pr_check(g, good) < 1e-6
+    # one score raised by 1%, two scores swapped, all scores of the
+    # undamped iteration: each is flagged far above the reference's 1e-4
+    bumped = good.copy()
+    v = int(np.argmax(good))
+    bumped[v] *= np.float32(1.01)
+    assert O.pr_check(g, bumped) > 1e-4
+    swapped = good.copy()
+    a, b = int(np.argmax(good)), int(np.argmin(good))
+    swapped[a], swapped[b] = good[b], good[a]
+    assert O.pr_check(g, swapped) > 1e-4
+    uniform = np.full(g.nodes, 1.0 / g.nodes, np.float32)
+    assert O.pr_check(g, uniform) > 1e-4
+    # and it is the residual of one more iteration, term by term
+    for sc in (good, bumped, swapped, uniform):
+        assert abs(O.pr_check(g, sc) - _pr_check_py(g, sc)) <= 1e-5 * max(1.0, _pr_check_py(g, sc))
+
+
+def test_pr_check_star_closed_form():
+    """Uniform scores 1/n on a star K_{1,k}: the centre's next score is
+    (1-d)/n + d*k/n, each leaf's is (1-d)/n + d/(k n); the check returns the
+    sum of the k + 1 absolute changes."""
+    k = 9
+    n = k + 1
+    d = 0.85
+    g = G.from_edges(n, np.zeros(k, np.int32), np.arange(1, n))
+    sc = np.full(n, 1.0 / n, np.float32)
+    want = abs((1 - d) / n + d * k / n - 1.0 / n) + k * abs((1 - d) / n + d / (k * n) - 1.0 / n)
+    assert abs(O.pr_check(g, sc) - want) < 1e-6
+
+
 # ------------------------------------------------------ PR closed forms ----
 def test_pr_cycle_is_uniform():
     n = 1000
