@@ -27,7 +27,8 @@ OpenMP styles (SSSP and CC vertex-based topology-driven pull) on the same
 R-MAT 22 graph.  Beside it, the style-matched restatements (SSSP edge-based
 data-driven push and CC vertex-based topology-driven push, both serialised by
 the reference's global critical section) on a bounded sample: the same
-generator at scale 16.
+generator at scale 16; and the C++-threads restatement (codes/cpp family) on
+the same R-MAT 22 graph.
 
 selector: on the bench graph, for SSSP and CC, the learned selector's pick,
 the library default and the exhaustive sweep's best style, each timed.
@@ -191,6 +192,25 @@ def cpu_style_matched(scale=16, reps=3):
     return {"value": round(2 * g.edges / (t1 + t2) * 1e-9, 5), "unit": "GTEPS",
             "sample": (f"R-MAT scale {scale} (n={g.nodes}, m={g.edges}): SSSP_OMP_E_Data_Push restatement "
                        f"{t1:.3f} s + CC_OMP_V_Topo_Push restatement {t2:.3f} s, median of {reps}")}
+
+
+def cpu_cpp_threads(g, src, reps=3):
+    """The oracle's C++-threads restatement of the reference's codes/cpp
+    styles (SSSP vertex topology-driven pull, CC vertex topology-driven push,
+    std::thread per round over a block partition) on the bench graph, median
+    of `reps` runs each, as many threads as OpenMP uses."""
+    import oracle as O
+    threads = O.num_threads()
+    ts, tc = [], []
+    for _ in range(reps):
+        _, t1, its = O.cpp_sssp(g, src, threads)
+        _, t2, itc = O.cpp_cc(g, threads)
+        ts.append(t1)
+        tc.append(t2)
+    t1, t2 = float(np.median(ts)), float(np.median(tc))
+    return {"value": round(2 * g.edges / (t1 + t2) * 1e-9, 4), "unit": "GTEPS", "threads": threads,
+            "sample": (f"same R-MAT graph: SSSP_CPP_V_Topo_Pull restatement {t1:.3f} s ({its} iterations) + "
+                       f"CC_CPP_V_Topo_Push restatement {t2:.3f} s ({itc} iterations), median of {reps}")}
 
 
 def selector_section(D, dg, g, src, algos=("sssp", "cc")):
@@ -388,6 +408,7 @@ def main():
         t = time.time()
         cpu = cpu_baseline(args.cpu_scale, g, src)
         cpu["style_matched"] = cpu_style_matched()
+        cpu["cpp_threads"] = cpu_cpp_threads(g, src)
         log(f"cpu baseline in {time.time() - t:.1f}s")
 
     line = {

@@ -1,8 +1,9 @@
 """ORACLE -- test infrastructure, not product code.
 
-Python access to the C++/OpenMP restatement of the reference's CPU path
-(oracle/oracle.cpp -> liboracle.so) and to the reference's own OpenMP
-variants compiled in place into oracle/_ref/ (present only where the
+Python access to the C++/OpenMP and C++-threads restatements of the
+reference's CPU path (oracle/oracle.cpp and oracle/oracle_cpp.cpp ->
+liboracle.so) and to the reference's own OpenMP and C++-threads variants
+compiled in place into oracle/_ref/ (present only where the
 reference tree was available at build time).  Only tests/, the smoke check
 and bench.py's CPU-baseline leg use this package, and only as the checker or
 as the timed CPU baseline.
@@ -69,6 +70,13 @@ def lib():
         sig("orc_tc_omp_vertex", ctypes.c_double, [G, ullp])
         sig("orc_tc_omp_edge", ctypes.c_double, [G, ip, ullp])
         sig("orc_num_threads", ctypes.c_int, [])
+        ci = ctypes.c_int
+        sig("orc_cpp_bfs_topo_push", ctypes.c_double, [ci, G, ip, ci, ip])
+        sig("orc_cpp_sssp_topo_pull", ctypes.c_double, [ci, G, ip, ci, ip])
+        sig("orc_cpp_cc_topo_push", ctypes.c_double, [G, ip, ci, ip])
+        sig("orc_cpp_mis_topo_push", ctypes.c_double, [G, up, bp, ci, ip])
+        sig("orc_cpp_pr_pull", ctypes.c_double, [G, fp, ctypes.c_double, ci, ci, ip])
+        sig("orc_cpp_tc_vertex", ctypes.c_double, [G, ullp, ci])
         _lib = L
     return _lib
 
@@ -221,6 +229,54 @@ def num_threads():
     return lib().orc_num_threads()
 
 
+# ------------------------------------------- C++-threads restatement ----
+# oracle_cpp.cpp: the reference's codes/cpp variants ([P-BFS], [P-SSSP],
+# [P-CC], [P-MIS], [P-PR], [P-TC] in that file's header); each returns
+# (result, seconds[, iterations]).
+def cpp_bfs(g, src, threads=4):
+    out = np.empty(g.nodes, np.int32)
+    it = ctypes.c_int()
+    t = lib().orc_cpp_bfs_topo_push(src, ctypes.byref(_gs(g)), _ip(out), threads, ctypes.byref(it))
+    return out, t, it.value
+
+
+def cpp_sssp(g, src, threads=4):
+    out = np.empty(g.nodes, np.int32)
+    it = ctypes.c_int()
+    t = lib().orc_cpp_sssp_topo_pull(src, ctypes.byref(_gs(g)), _ip(out), threads, ctypes.byref(it))
+    return out, t, it.value
+
+
+def cpp_cc(g, threads=4):
+    out = np.empty(g.nodes, np.int32)
+    it = ctypes.c_int()
+    t = lib().orc_cpp_cc_topo_push(ctypes.byref(_gs(g)), _ip(out), threads, ctypes.byref(it))
+    return out, t, it.value
+
+
+def cpp_mis(g, threads=4):
+    prio = np.empty(g.nodes, np.uint32)
+    st = np.empty(g.nodes, np.uint8)
+    it = ctypes.c_int()
+    t = lib().orc_cpp_mis_topo_push(ctypes.byref(_gs(g)), prio.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                                    st.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), threads, ctypes.byref(it))
+    return st, t, it.value
+
+
+def cpp_pr(g, eps=1e-4, max_iter=100, threads=4):
+    sc = np.full(g.nodes, np.float32(1.0) / np.float32(g.nodes), np.float32)
+    it = ctypes.c_int()
+    t = lib().orc_cpp_pr_pull(ctypes.byref(_gs(g)), sc.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), eps,
+                              max_iter, threads, ctypes.byref(it))
+    return sc, t, it.value
+
+
+def cpp_tc(g, threads=4):
+    c = ctypes.c_ulonglong()
+    t = lib().orc_cpp_tc_vertex(ctypes.byref(_gs(g)), ctypes.byref(c), threads)
+    return int(c.value), t
+
+
 # ---------------------------------------------------------------- _ref ----
 REF_OUT = {  # name -> (dtype, is_scalar)
     "bfs_push": (np.int32, False), "bfs_pull": (np.int32, False),
@@ -228,6 +284,8 @@ REF_OUT = {  # name -> (dtype, is_scalar)
     "cc_push": (np.int32, False), "cc_pull": (np.int32, False),
     "mis_vertex": (np.uint8, False), "mis_edge": (np.uint8, False),
     "pr_pull": (np.float32, False), "tc_vertex": (np.int64, True), "tc_edge": (np.int64, True),
+    "cpp_bfs": (np.int32, False), "cpp_sssp": (np.int32, False), "cpp_cc": (np.int32, False),
+    "cpp_mis": (np.uint8, False), "cpp_pr": (np.float32, False), "cpp_tc": (np.int64, True),
 }
 
 
@@ -236,7 +294,9 @@ def ref_available(name: str) -> bool:
 
 
 def ref_run(name: str, g, src: int = 0, threads: int = 4):
-    """Run the reference's own OpenMP variant `name` on g; returns its output array."""
+    """Run the reference's own OpenMP or C++-threads variant `name` on g
+    (OMP_NUM_THREADS, or the thread-count argument, = threads); returns its
+    output array."""
     exe = os.path.join(REF_DIR, "ref_" + name)
     dtype, scalar = REF_OUT[name]
     with tempfile.TemporaryDirectory() as td:
@@ -244,6 +304,6 @@ def ref_run(name: str, g, src: int = 0, threads: int = 4):
         out = os.path.join(td, "out.bin")
         g.write_egr(gin)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        subprocess.run([exe, gin, out, str(src)], check=True, env=env, stdout=subprocess.DEVNULL)
+        subprocess.run([exe, gin, out, str(src), str(threads)], check=True, env=env, stdout=subprocess.DEVNULL)
         data = np.fromfile(out, dtype=dtype)
     return int(data[0]) if scalar else data

@@ -1,6 +1,6 @@
 // ORACLE -- test infrastructure, not product code.
 //
-// Dump driver for the reference's own OpenMP variants.  oracle/Makefile
+// Dump driver for the reference's own OpenMP and C++-threads variants.  oracle/Makefile
 // compiles this file once per variant with -DREF_FILE="<variant path under
 // the reference tree>" and -DREF_KIND=<n>.  The variant's main() is renamed
 // away; this main() reads an .egr file, calls the variant's entry point once
@@ -8,7 +8,7 @@
 // restatement in oracle.cpp with the reference code itself.  Nothing is copied
 // from the reference: the variant is included from where it lies.
 //
-// usage: ref_<name> input.egr output.bin [source]
+// usage: ref_<name> input.egr output.bin [source [threads]]
 #define main indigo_ref_main
 #include REF_FILE
 #undef main
@@ -23,6 +23,8 @@ static void dump(const char* path, const void* data, size_t bytes)
   if (bytes > 0 && fwrite(data, 1, bytes, f) != bytes) {fprintf(stderr, "short write\n"); exit(2);}
   fclose(f);
 }
+
+static int ref_threads(int argc, char* argv[]) { return (argc > 4) ? atoi(argv[4]) : 4; }
 
 static int* make_sp(const ECLgraph& g)
 {
@@ -84,6 +86,37 @@ int main(int argc, char* argv[])
   int* sp = make_sp(g);
   basic_t count = 0;
   CPUtc_edge(count, g.edges, g.nindex, g.nlist, sp);
+  long long c = count;
+  dump(argv[2], &c, sizeof(c));
+/* C++-threads variants (codes/cpp): the entry takes a thread count, argv[4] */
+#elif REF_KIND == 10 /* BFS: CPPbfs_vertex(src, g, dist, threads) */
+  data_type* out = new data_type [g.nodes];
+  CPPbfs_vertex(src, g, out, ref_threads(argc, argv));
+  dump(argv[2], out, sizeof(data_type) * g.nodes);
+#elif REF_KIND == 11 /* SSSP: CPUsssp_vertex(src, g, dist, threads) */
+  if (g.eweight == NULL) {fprintf(stderr, "weights required\n"); return 2;}
+  data_type* out = new data_type [g.nodes];
+  CPUsssp_vertex(src, g, out, ref_threads(argc, argv));
+  dump(argv[2], out, sizeof(data_type) * g.nodes);
+#elif REF_KIND == 12 /* CC: CPUcc_vertex(g, label, threads) */
+  data_type* out = new data_type [g.nodes];
+  CPUcc_vertex(g, out, ref_threads(argc, argv));
+  dump(argv[2], out, sizeof(data_type) * g.nodes);
+#elif REF_KIND == 13 /* MIS: CPPmis_vertex(g, priority, status, threads) */
+  data_type* prio = new data_type [g.nodes];
+  unsigned char* out = new unsigned char [g.nodes];
+  CPPmis_vertex(g, prio, out, ref_threads(argc, argv));
+  dump(argv[2], out, g.nodes);
+#elif REF_KIND == 14 /* PR: PR_CPU(g, scores, degree, threads), scores start at 1/n */
+  int* degree = new int [g.nodes];
+  for (int i = 0; i < g.nodes; i++) degree[i] = g.nindex[i + 1] - g.nindex[i];
+  score_type* out = new score_type [g.nodes];
+  for (int i = 0; i < g.nodes; i++) out[i] = 1.0f / (score_type)g.nodes;
+  PR_CPU(g, out, degree, ref_threads(argc, argv));
+  dump(argv[2], out, sizeof(score_type) * g.nodes);
+#elif REF_KIND == 15 /* TC: CPUtc_vertex(count, nodes, nindex, nlist, threads) */
+  basic_t count = 0;
+  CPUtc_vertex(count, g.nodes, g.nindex, g.nlist, ref_threads(argc, argv));
   long long c = count;
   dump(argv[2], &c, sizeof(c));
 #else

@@ -3,7 +3,8 @@
 Run in a checkout where the reference tree is present and `make -C oracle ref`
 has built oracle/_ref/.  For each seeded graph below this script stores the
 CSR (plus weights) and the outputs of the reference variants compiled from
-codes/omp (BFS, SSSP, CC, MIS, PR with the reference's EPSILON = 1e-4, TC).
+codes/omp and codes/cpp (BFS, SSSP, CC, MIS, PR with the reference's
+EPSILON = 1e-4, TC).
 The fixtures are data only; the GPU box uses them without the reference.
 
 usage: python tests/golden/make_golden.py
@@ -42,6 +43,13 @@ def main():
             "pr_eps1e-4": O.ref_run("pr_pull", g),
             "tc": np.int64(O.ref_run("tc_vertex", g)),
             "tc_edge": np.int64(O.ref_run("tc_edge", g)),
+            # C++-threads variants (codes/cpp), 4 threads
+            "cpp_bfs": O.ref_run("cpp_bfs", g, src),
+            "cpp_sssp": O.ref_run("cpp_sssp", g, src),
+            "cpp_cc": O.ref_run("cpp_cc", g),
+            "cpp_mis": O.ref_run("cpp_mis", g),
+            "cpp_pr_eps1e-4": O.ref_run("cpp_pr", g),
+            "cpp_tc": np.int64(O.ref_run("cpp_tc", g)),
         }
         path = os.path.join(HERE, f"seeded_{name}.npz")
         np.savez_compressed(path, **out)

@@ -117,6 +117,81 @@ def test_oracle_matches_reference_binaries_fresh(seed):
     assert O.tc_serial(g) == O.ref_run("tc_edge", g)
 
 
+# ------------------------------------- C++-threads restatement ----
+@pytest.mark.parametrize("name", SEEDED)
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_cpp_restatement_matches_reference_golden(name, threads):
+    """oracle_cpp.cpp against the outputs of the reference's own codes/cpp
+    variants (4 threads) stored in the golden fixtures, at several thread
+    counts (the fixed points do not depend on the partition)."""
+    g, d = seeded(name)
+    src = int(d["src"])
+    assert np.array_equal(O.cpp_bfs(g, src, threads)[0], d["cpp_bfs"])
+    assert np.array_equal(O.cpp_sssp(g, src, threads)[0], d["cpp_sssp"])
+    assert np.array_equal(O.cpp_cc(g, threads)[0], d["cpp_cc"])
+    assert np.array_equal(O.cpp_mis(g, threads)[0], d["cpp_mis"])
+    assert O.cpp_tc(g, threads)[0] == int(d["cpp_tc"])
+    # the same answers as the OpenMP family: the C++-threads MIS tie rule
+    # ([P-MIS]:63) and the OpenMP one select the same set
+    assert np.array_equal(d["cpp_bfs"], d["bfs"]) and np.array_equal(d["cpp_sssp"], d["sssp"])
+    assert np.array_equal(d["cpp_cc"], d["cc"]) and np.array_equal(d["cpp_mis"], d["mis"])
+    # PageRank: FP32 per-vertex sums in neighbour order as in the reference,
+    # so the scores are bit-identical; only the residual's summation order
+    # depends on the threads
+    sc, _, it = O.cpp_pr(g, 1e-4, 100, threads)
+    assert np.array_equal(sc, d["cpp_pr_eps1e-4"])
+    assert np.array_equal(d["cpp_pr_eps1e-4"], d["pr_eps1e-4"])
+
+
+@pytest.mark.parametrize("fx", small_fixtures(), ids=lambda f: str(f["id"]))
+def test_cpp_restatement_known_answers(fx):
+    g = fixture_graph(fx).with_reference_weights()
+    assert O.cpp_bfs(g, 0)[0].tolist() == _vals(fx["bfs"]).tolist()
+    assert O.cpp_sssp(g, 0)[0].tolist() == _vals(fx["sssp"]).tolist()
+    assert O.cpp_cc(g)[0].tolist() == fx["cc"]
+    st = O.cpp_mis(g)[0]
+    assert sorted(np.nonzero(st == 1)[0].tolist()) == fx["mis"]
+    assert O.cpp_tc(g)[0] == fx["tc"]
+
+
+def test_cpp_restatement_iteration_counts_and_tie_rule():
+    """Topology-driven rounds: on one thread, which walks the vertices in
+    increasing order, BFS on a path of k edges settles the whole path in one
+    round from vertex 0 (the updates propagate within the sweep) but only one
+    level per round from vertex k, and needs one more round to see no change;
+    the MIS
+    tie rule prefers the higher id among equal priorities, which the hash
+    never produces, so a star's centre joins exactly when it outranks every
+    leaf."""
+    k = 50
+    path = G.from_edges(k + 1, np.arange(k), np.arange(1, k + 1))
+    d, _, it = O.cpp_bfs(path, 0, 1)
+    assert d.tolist() == list(range(k + 1)) and it == 2
+    d, _, it = O.cpp_bfs(path, k, 1)
+    assert d.tolist() == list(range(k, -1, -1)) and it == k + 1
+    star = G.from_edges(9, np.zeros(8, np.int32), np.arange(1, 9))
+    st = O.cpp_mis(star)[0]
+    p = [O.mis_hash(v + 712313887) for v in range(9)]
+    centre_wins = all(p[0] > p[v] for v in range(1, 9))
+    assert (st[0] == 1) == centre_wins and O.mis_verify(star, st) == 0
+
+
+@pytest.mark.skipif(not reference_present(), reason="reference tree not mounted")
+@pytest.mark.parametrize("seed", [31, 32])
+def test_cpp_restatement_matches_reference_binaries_fresh(seed):
+    if not O.ref_available("cpp_bfs"):
+        pytest.skip("oracle/_ref not built")
+    g = G.chung_lu(3000, 12, 2.2, seed=seed).with_weights(1, 255, seed=seed)
+    src = g.max_degree_vertex()
+    for threads in (1, 6):
+        assert np.array_equal(O.cpp_bfs(g, src, threads)[0], O.ref_run("cpp_bfs", g, src, threads))
+        assert np.array_equal(O.cpp_sssp(g, src, threads)[0], O.ref_run("cpp_sssp", g, src, threads))
+        assert np.array_equal(O.cpp_cc(g, threads)[0], O.ref_run("cpp_cc", g, 0, threads))
+        assert np.array_equal(O.cpp_mis(g, threads)[0], O.ref_run("cpp_mis", g, 0, threads))
+        assert np.array_equal(O.cpp_pr(g, 1e-4, 100, threads)[0], O.ref_run("cpp_pr", g, 0, threads))
+        assert O.cpp_tc(g, threads)[0] == O.ref_run("cpp_tc", g, 0, threads)
+
+
 # --------------------------------------------------------- textbook ----
 @pytest.mark.parametrize("make", [lambda: G.erdos_renyi(16384, 8, seed=1), lambda: G.rmat(13, 16, seed=4),
                                   lambda: G.grid2d(50, 0.05, seed=2)])
