@@ -189,6 +189,16 @@ int igx_batch_graph_view(const igx_batch* b, int i, igx_graph* view);
  * replays it (re-captured when the batch, mask, block or outputs change). */
 int igx_batch_run(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int* d_mis, int block, int use_graph,
                   igx_run_info* info);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same as igx_batch_run with use_graph = 0, but the counters and
+ * per-vertex scratch come from a fixed device array of the library (about
+ * 16 MiB per GPU, held until the process ends): batches of at most 2^20
+ * vertices in all (IGX_ERR_LIMIT beyond).  Allocates nothing; igx_init must
+ * have been called (IGX_ERR_ARG otherwise).  Calls on one GPU share that
+ * array, so calls that overlap on one GPU race. */
+int igx_batch_run_noalloc(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int* d_mis, int block,
+                          igx_run_info* info);
 
 /* ------------------------------------------------------ selector network ---- */
 /* No reference counterpart.  MLP in -> 64 -> 64 -> out (ReLU, FP32) trained

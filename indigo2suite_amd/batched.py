@@ -51,6 +51,8 @@ def _lib():
         lib.igx_batch_run.restype = ctypes.c_int
         lib.igx_batch_run.argtypes = [B, ctypes.c_int, ip, ip, ip, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(IgxRunInfo)]
+        lib.igx_batch_run_noalloc.restype = ctypes.c_int
+        lib.igx_batch_run_noalloc.argtypes = [B, ctypes.c_int, ip, ip, ip, ctypes.c_int, ctypes.POINTER(IgxRunInfo)]
         lib._batch_sigs = True
     return lib
 
@@ -88,6 +90,15 @@ class DeviceBatch:
         p = {k: ctypes.cast(v.ptr, ctypes.POINTER(ctypes.c_int)) for k, v in self.outs.items()}
         check(_lib().igx_batch_run(ctypes.byref(self.b), mask, p["bfs"], p["cc"], p["mis"], block, int(use_graph),
                                    ctypes.byref(info)), "igx_batch_run")
+        return info
+
+    def run_noalloc(self, mask: int = 7, block: int = 256) -> IgxRunInfo:
+        """igx_batch_run_noalloc: eager run with the library's fixed scratch
+        array (no device allocation; at most 2^20 vertices in the batch)."""
+        info = IgxRunInfo()
+        p = {k: ctypes.cast(v.ptr, ctypes.POINTER(ctypes.c_int)) for k, v in self.outs.items()}
+        check(_lib().igx_batch_run_noalloc(ctypes.byref(self.b), mask, p["bfs"], p["cc"], p["mis"], block,
+                                           ctypes.byref(info)), "igx_batch_run_noalloc")
         return info
 
     def results(self, algo: str) -> List[np.ndarray]:

@@ -427,6 +427,21 @@ bool same_batch(const igx_batch& a, const igx_batch& b)
 
 }  // namespace
 
+// ---------------------------------------------- allocation-free variant ----
+// igx_batch_run_noalloc takes its counters and per-vertex scratch from this
+// file-scope device array instead of allocating them: batches of at most
+// kNoAllocMaxNodes vertices in all.  Every GPU the library runs on holds its
+// own copy (about 16 MiB) until the process ends.  Host code never uses the
+// array's name (a host-side stand-in, not a device address): igx_init looks
+// up its device address once per device with hipGetSymbolAddress and the
+// entry point passes that address to the kernels.
+constexpr long long kNoAllocMaxNodes = 1LL << 20;
+constexpr size_t kNoAllocInts = kCounterInts + 4 * (size_t)kNoAllocMaxNodes + 64;
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+constexpr int kMaxDevices = 64;
+int* g_scratch_ptr[kMaxDevices] = {};
+
 int batch_release()
 {
   drop_graph();
@@ -437,6 +452,16 @@ int batch_release()
   g_state.counters = nullptr;
   g_state.scratch = nullptr;
   g_state.scratch_ints = 0;
+  return IGX_OK;
+}
+
+int batch_noalloc_setup(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return IGX_ERR_ARG;
+  if (g_scratch_ptr[device] != nullptr) return IGX_OK;
+  void* ptr = nullptr;
+  IGX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = (int*)ptr;
   return IGX_OK;
 }
 
@@ -522,6 +547,40 @@ int igx_batch_graph_view(const igx_batch* b, int i, igx_graph* view)
   view->nindex = const_cast<int*>(b->nindex) + no[0] + i;
   view->nlist = const_cast<int*>(b->nlist) + eo[0];
   view->eweight = nullptr;
+  return IGX_OK;
+}
+
+int igx_batch_run_noalloc(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int* d_mis, int block,
+                          igx_run_info* info)
+{
+  // no ensure_init(): initialising would allocate
+  Runtime& R = rt();
+  if (!R.ready) {
+    snprintf_last_error("igx_batch_run_noalloc: call igx_init first (the entry allocates nothing, not even at "
+                        "initialisation)");
+    return IGX_ERR_ARG;
+  }
+  if (b == nullptr || b->graphs < 1 || (algo_mask & ~7) != 0 || algo_mask == 0) return IGX_ERR_ARG;
+  if (((algo_mask & 1) && !d_bfs) || ((algo_mask & 2) && !d_cc) || ((algo_mask & 4) && !d_mis)) return IGX_ERR_ARG;
+  if (block != 256 && block != 512 && block != 1024) return IGX_ERR_ARG;
+  if (b->total_nodes > kNoAllocMaxNodes) {
+    snprintf_last_error("igx_batch_run_noalloc: the batch has %lld vertices; the allocation-free entry holds at "
+                        "most %lld (use igx_batch_run)", b->total_nodes, kNoAllocMaxNodes);
+    return IGX_ERR_LIMIT;
+  }
+  int* ptr = g_scratch_ptr[R.device];
+  if (ptr == nullptr) {
+    snprintf_last_error("igx_batch_run_noalloc: no scratch address for device %d", R.device);
+    return IGX_ERR_ARG;
+  }
+  int* outs[3] = {d_bfs, d_cc, d_mis};
+  igx_run_info local = {};
+  IGX_TRY(timer_start());
+  IGX_TRY(enqueue_with(b, algo_mask, outs, block, ptr, ptr + kCounterInts, R.stream));
+  IGX_TRY(timer_stop(&local.seconds));
+  local.launches = 1 + ((algo_mask & 1) != 0) + ((algo_mask & 2) != 0) + ((algo_mask & 4) != 0);
+  local.iterations = b->graphs;
+  if (info) *info = local;
   return IGX_OK;
 }
 

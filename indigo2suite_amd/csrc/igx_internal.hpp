@@ -156,6 +156,13 @@ int device_sp(const igx_graph* g, const int** d_sp);
 // Free the batched mode's own buffers and its captured graph.
 int batch_release();
 
+// Look up the device address of the allocation-free batched entry's scratch
+// array on `device` (the current device); called once per device by igx_init.
+int batch_noalloc_setup(int device);
+
+// printf-style message for igx_last_error()
+void snprintf_last_error(const char* fmt, ...);
+
 // Enqueue the construction of the chunk list of every row longer than the
 // frontier chunk (igx_frontier.hpp): bigq receives the chunks, *d_count their
 // number.  Two kernels, no atomics: per-block counts, then each block writes

@@ -3,6 +3,7 @@
 // reference-signature entry points.
 #include <hip/hip_runtime.h>
 
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -70,6 +71,14 @@ __global__ void build_sp_kernel(const int* __restrict__ nindex, int nodes, int* 
 void set_last_hip_error(hipError_t e, const char* what)
 {
   snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+}
+
+void snprintf_last_error(const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
 }
 
 Runtime& rt() { return g_rt; }
@@ -445,6 +454,7 @@ int igx_init(int device)
   IGX_HIP(hipMemset(g_rt.d_ctl, 0, kCtlAllocBytes));
   IGX_HIP(hipHostMalloc((void**)&g_rt.h_ctl, sizeof(Ctl), hipHostMallocDefault));
   memset(g_rt.h_ctl, 0, sizeof(Ctl));
+  IGX_TRY(batch_noalloc_setup(device));
   g_rt.ready = true;
   init_entry_styles();
   return IGX_OK;
