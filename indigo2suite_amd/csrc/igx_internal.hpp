@@ -53,7 +53,12 @@ enum WsSlot {
 int ws_get(int slot, size_t bytes, void** ptr);
 
 // Enqueue round kernels in growing batches until the control block reports
-// done or max_rounds were enqueued.  `launch(r)` enqueues round r.
+// done or max_rounds were enqueued.  `launch(r)` enqueues round r.  Batches:
+// first_batch, then 4, 8, 16, ... 64.  A round enqueued after convergence
+// returns at once on the device but still costs its host-side submission
+// (about 10 us, more than its device time), so the batch after the first
+// stays small: a run that needs a few rounds more than the first batch wastes
+// at most three submissions instead of a doubled batch.
 // Returns IGX_OK, IGX_ERR_LIMIT when max_rounds did not converge, or an error.
 template <typename F>
 int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches, int first_batch = 2)
@@ -76,7 +81,7 @@ int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches, 
       if (launches) *launches = (int)launched;
       return IGX_ERR_LIMIT;
     }
-    batch = (batch < 64) ? batch * 2 : 64;
+    batch = (launched == k) ? 4 : ((batch < 64) ? batch * 2 : 64);
   }
   if (rounds_done) *rounds_done = R.h_ctl->iter;
   if (launches) *launches = (int)launched;

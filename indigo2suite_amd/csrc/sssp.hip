@@ -345,6 +345,16 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
 // rows visited (8 B offsets + 4 B distance).
 constexpr int kAlphaS = 8;
 constexpr long long kQueueEdgesS = 1 << 16;
+// A pull round whose frontier holds more than this percentage of the edges
+// reads every neighbour's distance instead of testing its frontier bit first:
+// a neighbour outside the frontier only offers a candidate its row has
+// already seen (the row took it when that neighbour last changed), so the
+// result is the same, and the test is a dependent random access that mostly
+// passes.
+#ifndef IGX_SSSP_ALLFRONT_PCT
+#define IGX_SSSP_ALLFRONT_PCT 25
+#endif
+constexpr long long kAllFrontPct = IGX_SSSP_ALLFRONT_PCT;
 constexpr int kSlice = 512;  // edges per streamed slice of a pull run
 
 enum { kStatFrontierEdges = 0, kStatScanned = 1, kStatRelaxed = 2, kStatRows = 3 };
@@ -384,7 +394,9 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
                                                          unsigned* __restrict__ bm, int words, int grab, Ctl* ctl)
 {
   __shared__ int s_excl[kWavesPerBlock][kWave];
-  __shared__ int s_cand[kWavesPerBlock][kSlice];
+  __shared__ int s_acc[kWavesPerBlock][kWave];
+  __shared__ int s_rows[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_head[kWavesPerBlock][kSlice / kWave];
   __shared__ Entry s_ent[kWavesPerBlock][kWave];
   __shared__ long long s_red[kWavesPerBlock * kStats];
   if (round_done(ctl)) return;
@@ -393,6 +405,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   const int in_b = ctl->in_bits;
   const bool out_q = (mode == 0) && (ctl->edges_next <= kQueueEdgesS);
   const long long settled = ctl->aux_ll[1];
+  const bool all_front = (mode == 1) && ctl->edges_next * 100 > m * kAllFrontPct;
   const int p = iter & 1;
   const SegQ q_in = p ? q1 : q0;
   const SegQ q_out = p ? q0 : q1;
@@ -511,7 +524,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       #pragma unroll
       for (int h = 0; h < kPair; h++) {
         #pragma unroll
-        for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && bit_of(cur, u[h][k]);
+        for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && (all_front || bit_of(cur, u[h][k]));
       }
       int du[kPair][kPer];
       #pragma unroll
@@ -550,13 +563,21 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     // 64-row tiles.  Consecutive rows of at most kChunk edges form runs whose
     // edges are contiguous in nlist; a run is streamed in slices of kSlice
     // edges: the wave loads kSlice / 64 edges per lane with independent loads
-    // (ids, frontier bits, distances, weights), parks the candidates in LDS,
-    // and each lane then folds its own row's part of the slice.
+    // (ids, frontier bits, distances, weights) and folds each candidate into
+    // its row's minimum in LDS with an LDS atomicMin.  A position's row is
+    // found without a search: the run's non-empty rows start at distinct
+    // edges, which the owners mark in a head bitmap of the slice (one bit per
+    // position, kSlice / 64 words); the number of heads up to a position,
+    // plus the heads of the run's earlier slices, indexes the run's list of
+    // non-empty rows in lane order.  Every lane does the same work per slice,
+    // whatever the row lengths.
     // The tiles are claimed dynamically, `grab` at a time: the blocks of group g
     // take the tiles g, g + kGroups, g + 2 kGroups, ... in turn (tile weights
     // vary by more than 4x on R-MAT and drift with the vertex id; the long
     // rows' chunks above by little), so waves finish together.
-    int* cbuf = s_cand[w];
+    int* racc = s_acc[w];
+    int* rlist = s_rows[w];
+    unsigned long long* heads = s_head[w];
     const int ntiles = (n + kWave - 1) / kWave;
     const int grp = my_group();
     int* t_next = &ctl_groups(ctl)[grp].tiles;
@@ -579,7 +600,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         }
         // rows at most `settled` are final (see above) and skip their edges
         const bool small = (v < n) && (re - rb <= kChunk) && dv > settled;
-        int acc = kInf;
+        racc[lane] = kInf;
         uint64_t todo = ballot(small);
         while (todo != 0) {  // uniform: one run of consecutive small rows
           const int first = __ffsll((unsigned long long)todo) - 1;
@@ -588,11 +609,23 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
           const int last = first + cnt - 1;
           todo &= (cnt + first >= kWave) ? ((1ull << first) - 1) : ~(((1ull << cnt) - 1) << first);
           const int r0 = __shfl(rb, first, kWave), r1 = __shfl(re, last, kWave);
-          const bool mine = lane >= first && lane <= last;
+          const bool own = lane >= first && lane <= last && re > rb;
+          const uint64_t om = ballot(own);
+          __builtin_amdgcn_wave_barrier();
+          if (own) rlist[mbcnt(om)] = lane;
           if (lane == 0) scanned += r1 - r0;
+          int before = 0;  // heads of the run's earlier slices
           constexpr int kPerLane = kSlice / kWave;
           for (int c0 = r0; c0 < r1; c0 += kSlice) {
             const int c1 = (r1 - c0 < kSlice) ? r1 : c0 + kSlice;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < kPerLane) heads[lane] = 0ull;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (own && rb >= c0 && rb < c1) {
+              const int q = rb - c0;
+              atomicOr(&heads[q >> 6], 1ull << (q & 63));
+            }
             int u[kPerLane];
             #pragma unroll
             for (int k = 0; k < kPerLane; k++) {
@@ -601,29 +634,33 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
             }
             bool hit[kPerLane];
             #pragma unroll
-            for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0 && bit_of(cur, u[k]);
+            for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0 && (all_front || bit_of(cur, u[k]));
             int du[kPerLane], wt[kPerLane];
             #pragma unroll
             for (int k = 0; k < kPerLane; k++) {
               du[k] = hit[k] ? dist[u[k]] : kInf;
-              wt[k] = hit[k] ? ew[c0 + k * kWave + lane] : 0;
-            }
-            __builtin_amdgcn_wave_barrier();
-            #pragma unroll
-            for (int k = 0; k < kPerLane; k++) {
-              relaxed += hit[k] ? 1 : 0;
-              cbuf[k * kWave + lane] = (du[k] != kInf) ? du[k] + wt[k] : kInf;
+              wt[k] = (du[k] != kInf) ? ew[c0 + k * kWave + lane] : 0;
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            if (mine) {
-              const int lo = (rb > c0) ? rb : c0;
-              const int hi = (re < c1) ? re : c1;
-              for (int i = lo; i < hi; i++) acc = min(acc, cbuf[i - c0]);
+            const uint64_t upto = ~0ull >> (kWave - 1 - lane);  // positions 0..lane of a word
+            int pk = before;
+            #pragma unroll
+            for (int k = 0; k < kPerLane; k++) {
+              relaxed += hit[k] ? 1 : 0;
+              const uint64_t hw = heads[k];
+              if (du[k] != kInf) {
+                const int row = rlist[pk + __popcll(hw & upto) - 1];
+                atomicMin(&racc[row], du[k] + wt[k]);
+              }
+              pk += __popcll(hw);
             }
-            __builtin_amdgcn_wave_barrier();
+            before = pk;
           }
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int acc = racc[lane];
         const int len = re - rb;
         const bool improved = small && len > 0 && acc < dv;
         if (improved) {
@@ -638,6 +675,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
           if (lane == 0 && (unsigned)mask != 0u) atomicOr(&next[v0 >> 5], (unsigned)mask);
           if (lane == 1 && (unsigned)(mask >> 32) != 0u) atomicOr(&next[(v0 >> 5) + 1], (unsigned)(mask >> 32));
         }
+        __builtin_amdgcn_wave_barrier();
       }
     }
   }
