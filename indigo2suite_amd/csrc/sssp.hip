@@ -494,9 +494,10 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
                        relax);
     }
   } else {
-    // rows longer than kChunk: one wave per chunk, two chunks per step, their
-    // 2 x kChunk edges loaded together, a wave minimum per chunk, and one
-    // atomicMin on each row's distance
+    // rows longer than kChunk (most of the edges of a power-law graph): one
+    // wave per chunk, two chunks per step, their 2 x kChunk edges loaded
+    // together, a wave minimum per chunk, and one atomicMin on each row's
+    // distance
     const int nbig = *bigq_count;
     constexpr int kPer = kChunk / kWave;
     constexpr int kPair = 2;
@@ -521,16 +522,36 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         }
       }
       bool hit[kPair][kPer];
-      #pragma unroll
-      for (int h = 0; h < kPair; h++) {
+      if (all_front) {  // uniform
         #pragma unroll
-        for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && (all_front || bit_of(cur, u[h][k]));
+        for (int h = 0; h < kPair; h++) {
+          #pragma unroll
+          for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0;
+        }
+      } else {
+        // the frontier words are loaded unconditionally (word 0 for lanes
+        // without an edge) so that the loads issue together
+        unsigned bw[kPair][kPer];
+        #pragma unroll
+        for (int h = 0; h < kPair; h++) {
+          #pragma unroll
+          for (int k = 0; k < kPer; k++) bw[h][k] = cur[(u[h][k] >= 0 ? u[h][k] : 0) >> 5];
+        }
+        #pragma unroll
+        for (int h = 0; h < kPair; h++) {
+          #pragma unroll
+          for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && ((bw[h][k] >> (u[h][k] & 31)) & 1u);
+        }
       }
-      int du[kPair][kPer];
+      // distances and weights of the hits, all issued before the first use
+      int du[kPair][kPer], wt[kPair][kPer];
       #pragma unroll
       for (int h = 0; h < kPair; h++) {
         #pragma unroll
-        for (int k = 0; k < kPer; k++) du[h][k] = hit[h][k] ? dist[u[h][k]] : kInf;
+        for (int k = 0; k < kPer; k++) {
+          du[h][k] = hit[h][k] ? dist[u[h][k]] : kInf;
+          wt[h][k] = hit[h][k] ? ew[en[h].beg + k * kWave + lane] : 0;
+        }
       }
       int cand[kPair];
       #pragma unroll
@@ -539,7 +560,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         #pragma unroll
         for (int k = 0; k < kPer; k++) {
           relaxed += hit[h][k] ? 1 : 0;
-          if (du[h][k] != kInf) cand[h] = min(cand[h], du[h][k] + ew[en[h].beg + k * kWave + lane]);
+          if (du[h][k] != kInf) cand[h] = min(cand[h], du[h][k] + wt[h][k]);
         }
       }
       #pragma unroll
@@ -633,13 +654,21 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
               u[k] = (e < c1) ? nlist[e] : -1;
             }
             bool hit[kPerLane];
-            #pragma unroll
-            for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0 && (all_front || bit_of(cur, u[k]));
+            if (all_front) {  // uniform
+              #pragma unroll
+              for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0;
+            } else {
+              unsigned bw[kPerLane];  // issued together, see the long rows above
+              #pragma unroll
+              for (int k = 0; k < kPerLane; k++) bw[k] = cur[(u[k] >= 0 ? u[k] : 0) >> 5];
+              #pragma unroll
+              for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0 && ((bw[k] >> (u[k] & 31)) & 1u);
+            }
             int du[kPerLane], wt[kPerLane];
             #pragma unroll
             for (int k = 0; k < kPerLane; k++) {
               du[k] = hit[k] ? dist[u[k]] : kInf;
-              wt[k] = (du[k] != kInf) ? ew[c0 + k * kWave + lane] : 0;
+              wt[k] = hit[k] ? ew[c0 + k * kWave + lane] : 0;
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
