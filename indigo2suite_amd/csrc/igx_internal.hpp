@@ -157,6 +157,8 @@ PrParams& pr_params();
 
 // device sp[] of the graph, built on the device into the WS_SP workspace
 int device_sp(const igx_graph* g, const int** d_sp);
+// enqueue the construction of sp[] (edge -> row) into `sp` (edges ints)
+int build_sp_async(const igx_graph* g, int* sp);
 
 // Free the batched mode's own buffers and its captured graph.
 int batch_release();

@@ -123,6 +123,14 @@ int ctl_reset(int n)
   return IGX_OK;
 }
 
+int build_sp_async(const igx_graph* g, int* sp)
+{
+  hipLaunchKernelGGL(build_sp_kernel, dim3(blocks_for((long long)g->nodes * kWave, kBlock)), dim3(kBlock), 0,
+                     g_rt.stream, g->nindex, g->nodes, sp);
+  IGX_HIP(hipGetLastError());
+  return IGX_OK;
+}
+
 int device_sp(const igx_graph* g, const int** d_sp)
 {
   // Rebuilt on every call: a cache keyed by device pointers goes stale when

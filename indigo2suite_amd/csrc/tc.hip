@@ -33,19 +33,19 @@ __global__ void tc_zero(unsigned long long* __restrict__ c, int* __restrict__ fl
   }
 }
 
-__global__ void __launch_bounds__(kBlock) tc_check_sorted(const int* __restrict__ nindex, const int* __restrict__ nlist,
-                                                           int n, unsigned mult, int* __restrict__ bad)
+// The sorted check of the reference harness (tc_vertex_cuda.h:157-165:
+// every list strictly increasing, so no duplicates either) as one pass over
+// the edges: edge e breaks it when it shares its row (sp, the edge -> row
+// map) with e - 1 and does not exceed it.
+__global__ void __launch_bounds__(kBlock) tc_check_sorted_sp(const int* __restrict__ sp, const int* __restrict__ nlist,
+                                                              int m, int* __restrict__ bad)
 {
-  const int waves = gridDim.x * kWavesPerBlock;
-  const int lane = lane_id();
+  const int stride = gridDim.x * kBlock;
   bool b = false;
-  for (int k = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n; k += waves) {
-    const int v = spread_vertex(k, n, mult);
-    for (int i = nindex[v] + 1 + lane; i < nindex[v + 1]; i += kWave) {
-      if (nlist[i - 1] >= nlist[i]) b = true;
-    }
+  for (int e = blockIdx.x * kBlock + threadIdx.x + 1; e < m; e += stride) {
+    if (sp[e] == sp[e - 1] && nlist[e - 1] >= nlist[e]) b = true;
   }
-  if (ballot(b) != 0 && lane == 0) store_relaxed(bad, 1);
+  if (ballot(b) != 0 && lane_id() == 0) store_relaxed(bad, 1);
 }
 
 __device__ __forceinline__ void wave_add_count(unsigned long long* out, unsigned long long c)
@@ -155,50 +155,34 @@ __device__ __forceinline__ bool rank_above(int du, int u, int dv, int v)
   return (du > dv) || (du == dv && u > v);
 }
 
-__global__ void __launch_bounds__(kBlock) tc_orient_count(const int* __restrict__ nindex,
-                                                           const int* __restrict__ nlist, int n, unsigned mult,
-                                                           int* __restrict__ ocount)
+// Orientation as flat passes over the edges: keep[e] = 1 when edge (v, u)
+// points from v to a higher-ranked u (rank: degree, then id); an exclusive
+// scan of keep gives every kept edge its slot in the oriented list, which
+// keeps the row order and each row's sorted order, and the scan at a row's
+// first edge is the row's start.  Edges m.. of keep are the scan's padding.
+__global__ void __launch_bounds__(kBlock) tc_orient_keep(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                          const int* __restrict__ sp, int m, int* __restrict__ keep)
 {
-  const int waves = gridDim.x * kWavesPerBlock;
-  const int lane = lane_id();
-  for (int k = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n; k += waves) {
-    const int v = spread_vertex(k, n, mult);
-    const int beg = nindex[v], end = nindex[v + 1];
-    const int dv = end - beg;
-    int c = 0;
-    for (int i = beg + lane; i < end; i += kWave) {
-      const int u = nlist[i];
-      c += rank_above(nindex[u + 1] - nindex[u], u, dv, v) ? 1 : 0;
-    }
-    c = wave_sum(c);
-    if (lane == 0) ocount[v] = c;
+  const int stride = gridDim.x * kBlock;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+    const int v = sp[e], u = nlist[e];
+    const int dv = nindex[v + 1] - nindex[v], du = nindex[u + 1] - nindex[u];
+    keep[e] = rank_above(du, u, dv, v) ? 1 : 0;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) keep[m] = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) tc_orient_fill(const int* __restrict__ nindex,
-                                                          const int* __restrict__ nlist, int n, unsigned mult,
-                                                          const int* __restrict__ onidx, int* __restrict__ olist)
+__global__ void __launch_bounds__(kBlock) tc_orient_scatter(const int* __restrict__ nindex,
+                                                             const int* __restrict__ nlist, const int* __restrict__ keep,
+                                                             const int* __restrict__ pos, int n, int m,
+                                                             int* __restrict__ onidx, int* __restrict__ olist)
 {
-  const int waves = gridDim.x * kWavesPerBlock;
-  const int lane = lane_id();
-  for (int k = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n; k += waves) {
-    const int v = spread_vertex(k, n, mult);
-    const int beg = nindex[v], end = nindex[v + 1];
-    const int dv = end - beg;
-    int out = onidx[v];
-    for (int base = beg; base < end; base += kWave) {
-      const int i = base + lane;
-      bool keep = false;
-      int u = 0;
-      if (i < end) {
-        u = nlist[i];
-        keep = rank_above(nindex[u + 1] - nindex[u], u, dv, v);
-      }
-      const uint64_t mask = ballot(keep);
-      if (keep) olist[out + mbcnt(mask)] = u;
-      out += __popcll(mask);
-    }
+  const int stride = gridDim.x * kBlock;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  for (int e = t; e < m; e += stride) {
+    if (keep[e]) olist[pos[e]] = nlist[e];
   }
+  for (int v = t; v <= n; v += stride) onidx[v] = pos[nindex[v]];
 }
 
 // Count per vertex v: every oriented edge (v, u) adds |N+(v) ∩ N+(u)|.  One
@@ -338,15 +322,20 @@ __global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restric
 }
 
 // ------------------------------------------------------------ host side ----
-int check_sorted(const igx_graph* g)
+// the reference harness's sorted check (before the timed region, as there);
+// sp: the edge -> row map, built here when the style has none
+int check_sorted(const igx_graph* g, const int* sp)
 {
   Runtime& R = rt();
   void* p = nullptr;
   IGX_TRY(ws_get(WS_H, sizeof(int) * 64, &p));
   int* bad = (int*)p;
+  if (sp == nullptr) IGX_TRY(device_sp(g, &sp));
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, (unsigned long long*)nullptr, bad);
-  hipLaunchKernelGGL(tc_check_sorted, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, g->nodes,
-                     spread_mult(g->nodes), bad);
+  if (g->edges > 1) {
+    hipLaunchKernelGGL(tc_check_sorted_sp, dim3(blocks_for(g->edges, kBlock)), dim3(kBlock), 0, R.stream, sp, g->nlist,
+                       g->edges, bad);
+  }
   int h = 0;
   IGX_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, R.stream));
   IGX_HIP(hipStreamSynchronize(R.stream));
@@ -358,7 +347,7 @@ int style_vertex(const igx_graph* g, int, void* out, const void*, igx_run_info* 
 {
   Runtime& R = rt();
   unsigned long long* count = (unsigned long long*)out;
-  IGX_TRY(check_sorted(g));
+  IGX_TRY(check_sorted(g, nullptr));
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
   hipLaunchKernelGGL((tc_vertex_merge<WAVE>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
@@ -376,7 +365,7 @@ int style_edge(const igx_graph* g, int, void* out, const void* aux, igx_run_info
   Runtime& R = rt();
   unsigned long long* count = (unsigned long long*)out;
   const int* sp = (const int*)aux;
-  IGX_TRY(check_sorted(g));
+  IGX_TRY(check_sorted(g, sp));
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
   hipLaunchKernelGGL(tc_edge_bsearch, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, sp,
@@ -394,29 +383,33 @@ int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info
   Runtime& R = rt();
   unsigned long long* count = (unsigned long long*)out;
   const int n = g->nodes;
-  IGX_TRY(check_sorted(g));
-  void *pcnt = nullptr, *pidx = nullptr, *plist = nullptr, *ptile = nullptr;
-  IGX_TRY(ws_get(WS_A, sizeof(int) * ((size_t)n + 1), &pcnt));
+  const int m = g->edges;
+  IGX_TRY(check_sorted(g, nullptr));
+  void *pidx = nullptr, *plist = nullptr, *ptile = nullptr, *pkeep = nullptr, *ppos = nullptr, *psp = nullptr;
   IGX_TRY(ws_get(WS_B, sizeof(int) * ((size_t)n + 1), &pidx));
   // an undirected graph has exactly edges/2 oriented edges; size for edges so
   // that an asymmetric input cannot overrun the list
-  IGX_TRY(ws_get(WS_C, sizeof(int) * ((size_t)g->edges + 64), &plist));
-  IGX_TRY(ws_get(WS_E, sizeof(int) * ((size_t)scan_tiles_for(n + 1) + 64), &ptile));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * ((size_t)m + 64), &plist));
+  IGX_TRY(ws_get(WS_E, sizeof(int) * ((size_t)scan_tiles_for(m + 1) + 64), &ptile));
+  IGX_TRY(ws_get(WS_F, sizeof(int) * ((size_t)m + 1), &pkeep));
+  IGX_TRY(ws_get(WS_G, sizeof(int) * ((size_t)m + 1), &ppos));
+  IGX_TRY(ws_get(WS_A, sizeof(int) * ((size_t)(m > 0 ? m : 1)), &psp));
   void* pclaim = nullptr;
   IGX_TRY(ws_get(WS_D, sizeof(int) * kGroups * kClaimStride, &pclaim));
-  int* ocount = (int*)pcnt;
   int* onidx = (int*)pidx;
   int* tiles = (int*)ptile;
+  int* keep = (int*)pkeep;
+  int* pos = (int*)ppos;
+  int* sp = (int*)psp;
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
-  const unsigned mult = spread_mult(n);
-  hipLaunchKernelGGL(tc_orient_count, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, mult,
-                     ocount);
-  // scan n+1 entries so that onidx[n] is the total; ocount[n] is zeroed first
-  hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, (unsigned long long*)nullptr, ocount + n);
-  exclusive_scan(ocount, onidx, n + 1, tiles, tiles + scan_tiles_for(n + 1), R.stream);
-  hipLaunchKernelGGL(tc_orient_fill, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n, mult,
-                     onidx, (int*)plist);
+  // the edge -> row map inside the timed region: it serves the orientation
+  IGX_TRY(build_sp_async(g, sp));
+  hipLaunchKernelGGL(tc_orient_keep, dim3(blocks_for((long long)m + 1, kBlock)), dim3(kBlock), 0, R.stream, g->nindex,
+                     g->nlist, (const int*)sp, m, keep);
+  exclusive_scan(keep, pos, m + 1, tiles, tiles + scan_tiles_for(m + 1), R.stream);
+  hipLaunchKernelGGL(tc_orient_scatter, dim3(blocks_for((long long)m + 1, kBlock)), dim3(kBlock), 0, R.stream,
+                     g->nindex, g->nlist, (const int*)keep, (const int*)pos, n, m, onidx, (int*)plist);
   hipLaunchKernelGGL(tc_clear, dim3(1), dim3(kBlock), 0, R.stream, (int*)pclaim, kGroups * kClaimStride);
   ktimer_begin("tc_oriented_count");
   // identity order within each group's range: neighbouring vertices share
