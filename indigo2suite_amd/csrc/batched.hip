@@ -1,7 +1,7 @@
 // Batched mode: BFS, CC and MIS over hundreds of uneven graphs packed into one
-// arena, one persistent launch per algorithm, optionally replayed as one
-// captured hipGraph.  (No reference counterpart: the reference runs one
-// program per graph.)
+// arena, in one persistent launch, optionally replayed as one captured
+// hipGraph.  (No reference counterpart: the reference runs one program per
+// graph.)
 //
 // Arena layout (device memory):
 //   node_off[G+1], edge_off[G+1]  int64 prefix offsets of the graphs
@@ -12,10 +12,11 @@
 //   outputs int32[sum(n_i)]       graph i's values start at node_off[i]
 //
 // Execution: a persistent grid; every workgroup repeatedly takes the next
-// graph from an atomic counter (graphs are ordered largest first, so the long
-// ones start early) and runs the whole algorithm on it with workgroup
-// barriers between rounds.  No workgroup ever waits for another, so every
-// wave reaches the loop exit once the counter passes G.
+// (graph, algorithm) item from an atomic counter (graphs ordered largest
+// first, so the long ones start early) and runs the whole algorithm on that
+// graph with workgroup barriers between rounds.  No workgroup ever waits for
+// another, so every wave reaches the loop exit once the counter passes the
+// item count.
 //
 // Per-graph algorithms (results identical to the single-graph styles):
 //   BFS from vertex 0: level-synchronous top-down; each level expands the
@@ -115,59 +116,60 @@ __device__ __forceinline__ int next_graph(int* counter, int* s_g)
   return k;
 }
 
-// ------------------------------------------------------------------ BFS ----
+// Workgroup-shared scratch of the per-graph routines (one set per workgroup;
+// the routines run one after another in a workgroup, never nested).
 template <int B>
-__global__ void __launch_bounds__(B) batched_bfs(Arena A, const int* __restrict__ order, int* counter,
-                                                  int* __restrict__ dist_all, int* __restrict__ queue_all)
+struct WgLds {
+  int excl[B], row[B], beg[B];
+  int w[B / kWave];
+  int next;
+};
+
+// ------------------------------------------------------------------ BFS ----
+// BFS of one graph from vertex 0 by the whole workgroup (see the file head).
+template <int B>
+__device__ void bfs_graph(const Arena& A, int g, int* __restrict__ dist_all, int* __restrict__ queue_all, WgLds<B>& L)
 {
-  __shared__ int s_excl[B], s_row[B], s_beg[B];
-  __shared__ int s_w[B / kWave];
-  __shared__ int s_g, s_next;
-  while (true) {
-    const int k = next_graph(counter, &s_g);
-    if (k >= A.graphs) break;
-    const int g = order[k];
-    int n;
-    const int* nidx;
-    const int* nl;
-    long long voff;
-    graph_view(A, g, n, nidx, nl, voff);
-    int* dist = dist_all + voff;
-    int* q0 = queue_all + 2 * voff;
-    int* q1 = q0 + n;
-    for (int v = threadIdx.x; v < n; v += B) dist[v] = (v == 0) ? 0 : kInf;
-    if (threadIdx.x == 0) q0[0] = 0;
+  int n;
+  const int* nidx;
+  const int* nl;
+  long long voff;
+  graph_view(A, g, n, nidx, nl, voff);
+  int* dist = dist_all + voff;
+  int* q0 = queue_all + 2 * voff;
+  int* q1 = q0 + n;
+  for (int v = threadIdx.x; v < n; v += B) dist[v] = (v == 0) ? 0 : kInf;
+  if (threadIdx.x == 0) q0[0] = 0;
+  __syncthreads();
+  int qsize = 1;
+  int level = 0;
+  while (qsize > 0) {
+    if (threadIdx.x == 0) L.next = 0;
     __syncthreads();
-    int qsize = 1;
-    int level = 0;
-    while (qsize > 0) {
-      if (threadIdx.x == 0) s_next = 0;
-      __syncthreads();
-      const int* qin = (level & 1) ? q1 : q0;
-      int* qout = (level & 1) ? q0 : q1;
-      const int nd = level + 1;
-      wg_expand<B>(qsize, nidx, [&](int i) { return qin[i]; }, s_excl, s_row, s_beg, s_w, [&](int, int e) {
-        const int u = nl[e];
-        bool disc = false;
-        if (dist[u] == kInf) {
-          int expected = kInf;
-          disc = __hip_atomic_compare_exchange_strong(&dist[u], &expected, nd, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-        const uint64_t mask = ballot(disc);
-        if (mask != 0) {
-          const int leader = __ffsll((unsigned long long)mask) - 1;
-          int base = 0;
-          if (lane_id() == leader) base = atomicAdd(&s_next, __popcll(mask));
-          base = __shfl(base, leader, kWave);
-          if (disc) qout[base + mbcnt(mask)] = u;
-        }
-      });
-      __syncthreads();
-      qsize = s_next;
-      level++;
-      __syncthreads();
-    }
+    const int* qin = (level & 1) ? q1 : q0;
+    int* qout = (level & 1) ? q0 : q1;
+    const int nd = level + 1;
+    wg_expand<B>(qsize, nidx, [&](int i) { return qin[i]; }, L.excl, L.row, L.beg, L.w, [&](int, int e) {
+      const int u = nl[e];
+      bool disc = false;
+      if (dist[u] == kInf) {
+        int expected = kInf;
+        disc = __hip_atomic_compare_exchange_strong(&dist[u], &expected, nd, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      const uint64_t mask = ballot(disc);
+      if (mask != 0) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(&L.next, __popcll(mask));
+        base = __shfl(base, leader, kWave);
+        if (disc) qout[base + mbcnt(mask)] = u;
+      }
+    });
+    __syncthreads();
+    qsize = L.next;
+    level++;
+    __syncthreads();
   }
 }
 
@@ -200,44 +202,35 @@ __device__ __forceinline__ void wg_unite(int* __restrict__ parent, int a, int b)
 }
 
 template <int B>
-__global__ void __launch_bounds__(B) batched_cc(Arena A, const int* __restrict__ order, int* counter,
-                                                 int* __restrict__ label_all)
+__device__ void cc_graph(const Arena& A, int g, int* __restrict__ label_all, WgLds<B>& L)
 {
-  __shared__ int s_excl[B], s_row[B], s_beg[B];
-  __shared__ int s_w[B / kWave];
-  __shared__ int s_g;
-  while (true) {
-    const int k = next_graph(counter, &s_g);
-    if (k >= A.graphs) break;
-    const int g = order[k];
-    int n;
-    const int* nidx;
-    const int* nl;
-    long long voff;
-    graph_view(A, g, n, nidx, nl, voff);
-    int* parent = label_all + voff;
-    for (int v = threadIdx.x; v < n; v += B) {
-      const int u0 = (nidx[v + 1] > nidx[v]) ? nl[nidx[v]] : v;
-      parent[v] = (u0 < v) ? u0 : v;
-    }
-    __syncthreads();
-    wg_expand<B>(n, nidx, [&](int i) { return i; }, s_excl, s_row, s_beg, s_w, [&](int v, int e) {
-      const int u = nl[e];
-      if (u < v) wg_unite(parent, v, u);
-    });
-    __syncthreads();
-    // read-only walks; each thread writes only its own vertex (see uf_compress)
-    for (int v = threadIdx.x; v < n; v += B) {
-      int x = parent[v];
-      int p = parent[x];
-      while (p != x) {
-        x = p;
-        p = parent[x];
-      }
-      parent[v] = x;
-    }
-    __syncthreads();
+  int n;
+  const int* nidx;
+  const int* nl;
+  long long voff;
+  graph_view(A, g, n, nidx, nl, voff);
+  int* parent = label_all + voff;
+  for (int v = threadIdx.x; v < n; v += B) {
+    const int u0 = (nidx[v + 1] > nidx[v]) ? nl[nidx[v]] : v;
+    parent[v] = (u0 < v) ? u0 : v;
   }
+  __syncthreads();
+  wg_expand<B>(n, nidx, [&](int i) { return i; }, L.excl, L.row, L.beg, L.w, [&](int v, int e) {
+    const int u = nl[e];
+    if (u < v) wg_unite(parent, v, u);
+  });
+  __syncthreads();
+  // read-only walks; each thread writes only its own vertex (see uf_compress)
+  for (int v = threadIdx.x; v < n; v += B) {
+    int x = parent[v];
+    int p = parent[x];
+    while (p != x) {
+      x = p;
+      p = parent[x];
+    }
+    parent[v] = x;
+  }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ MIS ----
@@ -247,69 +240,103 @@ __device__ __forceinline__ bool mis_outranks(uint32_t pu, int u, uint32_t pv, in
 }
 
 template <int B>
-__global__ void __launch_bounds__(B) batched_mis(Arena A, const int* __restrict__ order, int* counter,
-                                                  int* __restrict__ status_all, int* __restrict__ wl_all)
+__device__ void mis_graph(const Arena& A, int g, int* __restrict__ status_all, int* __restrict__ wl_all, WgLds<B>& L)
 {
-  __shared__ int s_g, s_next;
-  while (true) {
-    const int k = next_graph(counter, &s_g);
-    if (k >= A.graphs) break;
-    const int g = order[k];
-    int n;
-    const int* nidx;
-    const int* nl;
-    long long voff;
-    graph_view(A, g, n, nidx, nl, voff);
-    int* status = status_all + voff;
-    int* w0 = wl_all + 2 * voff;
-    int* w1 = w0 + n;
-    for (int v = threadIdx.x; v < n; v += B) {
-      status[v] = 0;
-      w0[v] = v;
+  int n;
+  const int* nidx;
+  const int* nl;
+  long long voff;
+  graph_view(A, g, n, nidx, nl, voff);
+  int* status = status_all + voff;
+  int* w0 = wl_all + 2 * voff;
+  int* w1 = w0 + n;
+  for (int v = threadIdx.x; v < n; v += B) {
+    status[v] = 0;
+    w0[v] = v;
+  }
+  __syncthreads();
+  int count = n;
+  int round = 0;
+  while (count > 0) {
+    if (threadIdx.x == 0) L.next = 0;
+    __syncthreads();
+    const int* win = (round & 1) ? w1 : w0;
+    int* wout = (round & 1) ? w0 : w1;
+    for (int base = 0; base < count; base += B) {
+      const int idx = base + (int)threadIdx.x;
+      bool keep = false;
+      int v = 0;
+      if (idx < count) {
+        v = win[idx];
+        const uint32_t pv = mis_priority(v);
+        int decision = 1;  // included unless blocked
+        for (int i = nidx[v]; i < nidx[v + 1]; i++) {
+          const int u = nl[i];
+          if (u == v) continue;
+          const int su = __hip_atomic_load(&status[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (su == 1) {
+            decision = 2;
+            break;
+          }
+          if (su != 2 && mis_outranks(mis_priority(u), u, pv, v)) decision = 0;
+        }
+        if (decision == 0) keep = true;
+        else __hip_atomic_store(&status[v], decision, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      const uint64_t mask = ballot(keep);
+      if (mask != 0) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int b0 = 0;
+        if (lane_id() == leader) b0 = atomicAdd(&L.next, __popcll(mask));
+        b0 = __shfl(b0, leader, kWave);
+        if (keep) wout[b0 + mbcnt(mask)] = v;
+      }
     }
     __syncthreads();
-    int count = n;
-    int round = 0;
-    while (count > 0) {
-      if (threadIdx.x == 0) s_next = 0;
-      __syncthreads();
-      const int* win = (round & 1) ? w1 : w0;
-      int* wout = (round & 1) ? w0 : w1;
-      for (int base = 0; base < count; base += B) {
-        const int idx = base + (int)threadIdx.x;
-        bool keep = false;
-        int v = 0;
-        if (idx < count) {
-          v = win[idx];
-          const uint32_t pv = mis_priority(v);
-          int decision = 1;  // included unless blocked
-          for (int i = nidx[v]; i < nidx[v + 1]; i++) {
-            const int u = nl[i];
-            if (u == v) continue;
-            const int su = __hip_atomic_load(&status[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (su == 1) {
-              decision = 2;
-              break;
-            }
-            if (su != 2 && mis_outranks(mis_priority(u), u, pv, v)) decision = 0;
-          }
-          if (decision == 0) keep = true;
-          else __hip_atomic_store(&status[v], decision, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    count = L.next;
+    round++;
+    __syncthreads();
+  }
+}
+
+// One persistent launch for every (graph, algorithm) pair of the batch: a
+// workgroup takes the next item from one counter, items ordered by graph
+// (largest first) and then BFS, CC, MIS, so the three algorithms of the
+// largest graphs run side by side on different workgroups from the start
+// instead of one algorithm's largest graph being every launch's tail.  The
+// per-vertex scratch of BFS (two queues) and of MIS (two worklists) are
+// disjoint halves of `scratch`, so a graph's BFS and MIS may run at once.
+template <int B>
+__global__ void __launch_bounds__(B) batched_all(Arena A, const int* __restrict__ order, int* counter, int mask,
+                                                  int* __restrict__ d_bfs, int* __restrict__ d_cc,
+                                                  int* __restrict__ d_mis, int* __restrict__ scratch,
+                                                  long long total_nodes)
+{
+  __shared__ WgLds<B> L;
+  __shared__ int s_k;
+  const int per = __popc((unsigned)mask);
+  const long long items = (long long)A.graphs * per;
+  while (true) {
+    if (threadIdx.x == 0) s_k = atomicAdd(counter, 1);
+    __syncthreads();
+    const int k = s_k;
+    __syncthreads();
+    if (k >= items) break;
+    const int g = order[k / per];
+    int which = k % per;  // the which-th algorithm of the mask
+    int algo = 0;
+    for (int a = 0; a < 3; a++) {
+      if ((mask >> a) & 1) {
+        if (which == 0) {
+          algo = a;
+          break;
         }
-        const uint64_t mask = ballot(keep);
-        if (mask != 0) {
-          const int leader = __ffsll((unsigned long long)mask) - 1;
-          int b0 = 0;
-          if (lane_id() == leader) b0 = atomicAdd(&s_next, __popcll(mask));
-          b0 = __shfl(b0, leader, kWave);
-          if (keep) wout[b0 + mbcnt(mask)] = v;
-        }
+        which--;
       }
-      __syncthreads();
-      count = s_next;
-      round++;
-      __syncthreads();
     }
+    if (algo == 0) bfs_graph<B>(A, g, d_bfs, scratch, L);
+    else if (algo == 1) cc_graph<B>(A, g, d_cc, L);
+    else mis_graph<B>(A, g, d_mis, scratch + 2 * total_nodes, L);
   }
 }
 
@@ -342,24 +369,21 @@ constexpr int kCounterInts = 64;
 // worklist pair (2 per vertex), and a small tail
 inline size_t scratch_ints_for(const igx_batch* b) { return 4 * (size_t)b->total_nodes + 64; }
 
-template <int B>
-void launch_all(const igx_batch* b, const Arena& A, int mask, int* const* outs, int* counters, int* scratch, int grid,
-                hipStream_t s)
-{
-  hipLaunchKernelGGL(batched_reset, dim3(1), dim3(64), 0, s, counters, 3);
-  if (mask & 1) hipLaunchKernelGGL((batched_bfs<B>), dim3(grid), dim3(B), 0, s, A, b->order, counters + 0, outs[0],
-                                   scratch);
-  if (mask & 2) hipLaunchKernelGGL((batched_cc<B>), dim3(grid), dim3(B), 0, s, A, b->order, counters + 1, outs[1]);
-  if (mask & 4) hipLaunchKernelGGL((batched_mis<B>), dim3(grid), dim3(B), 0, s, A, b->order, counters + 2, outs[2],
-                                   scratch + 2 * b->total_nodes);
-}
-
 void drop_graph()
 {
   if (g_state.exec) hipGraphExecDestroy(g_state.exec);
   if (g_state.graph) hipGraphDestroy(g_state.graph);
   g_state.exec = nullptr;
   g_state.graph = nullptr;
+}
+
+template <int B>
+void launch_all(const igx_batch* b, const Arena& A, int mask, int* const* outs, int* counters, int* scratch, int grid,
+                hipStream_t s)
+{
+  hipLaunchKernelGGL(batched_reset, dim3(1), dim3(64), 0, s, counters, 1);
+  hipLaunchKernelGGL((batched_all<B>), dim3(grid), dim3(B), 0, s, A, b->order, counters, mask, outs[0], outs[1],
+                     outs[2], scratch, (long long)b->total_nodes);
 }
 
 // Make the batch buffers hold at least `ints` scratch ints.  Growing them
@@ -400,7 +424,8 @@ int enqueue_with(const igx_batch* b, int mask, int* const* outs, int block, int*
   Arena A{b->node_off, b->edge_off, b->nindex, b->nlist, b->graphs};
   const int per_cu = R.max_threads_per_cu / block;
   int grid = R.cus * (per_cu > 0 ? per_cu : 1);
-  if (grid > b->graphs) grid = b->graphs;
+  const int items = b->graphs * __builtin_popcount((unsigned)mask);  // (graph, algorithm) pairs
+  if (grid > items) grid = items;
   if (grid < 1) grid = 1;
   switch (block) {
     case 256: launch_all<256>(b, A, mask, outs, counters, scratch, grid, s); break;
@@ -578,7 +603,7 @@ int igx_batch_run_noalloc(const igx_batch* b, int algo_mask, int* d_bfs, int* d_
   IGX_TRY(timer_start());
   IGX_TRY(enqueue_with(b, algo_mask, outs, block, ptr, ptr + kCounterInts, R.stream));
   IGX_TRY(timer_stop(&local.seconds));
-  local.launches = 1 + ((algo_mask & 1) != 0) + ((algo_mask & 2) != 0) + ((algo_mask & 4) != 0);
+  local.launches = 2;  // counter reset and the persistent launch
   local.iterations = b->graphs;
   if (info) *info = local;
   return IGX_OK;
@@ -598,7 +623,7 @@ int igx_batch_run(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int*
     IGX_TRY(timer_start());
     IGX_TRY(enqueue(b, algo_mask, outs, block, R.stream));
     IGX_TRY(timer_stop(&local.seconds));
-    local.launches = 1 + ((algo_mask & 1) != 0) + ((algo_mask & 2) != 0) + ((algo_mask & 4) != 0);
+    local.launches = 2;  // counter reset and the persistent launch
   } else {
     const bool same = same_batch(g_state.key, *b) && g_state.block == block &&
                       g_state.mask == algo_mask && g_state.outs[0] == d_bfs && g_state.outs[1] == d_cc &&

@@ -44,7 +44,7 @@ def test_noalloc_matches_batch_run_twice_then_smaller(gpu, block):
             _poison(db)
             _say("noalloc:", tag, "igx_batch_run_noalloc")
             info = db.run_noalloc(7, block)
-            assert info.launches == 4 and info.iterations == len(graphs)
+            assert info.launches == 2 and info.iterations == len(graphs)
             got = _outputs(db)
             for a in ("bfs", "cc", "mis"):
                 assert np.array_equal(got[a], want[a]), (tag, a)
