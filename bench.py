@@ -33,6 +33,10 @@ the same R-MAT 22 graph.
 selector: on the bench graph, for SSSP and CC, the learned selector's pick,
 the library default and the exhaustive sweep's best style, each timed.
 
+batched: the 512-graph batch (Erdos-Renyi and R-MAT halves, n log-uniform in
+[2^10, 2^18], degree U[4, 32]) as one captured persistent launch for BFS, CC
+and MIS, against one launch sequence per graph and algorithm.
+
 usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
 import argparse
@@ -151,7 +155,7 @@ def pr_tc_section(args, D, G):
     return out
 
 
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "sssp_do_round_rmat22_traffic.md")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "sssp_rmat22.md")
 
 
 def measured_traffic(scale):
@@ -267,6 +271,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-selector", action="store_true", help="skip the learned-pick vs default vs best section")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 512-graph batched-mode section")
     ap.add_argument("--no-extra", action="store_true", help="skip PageRank and TC on R-MAT 20 and the grid")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the known-size kernel that scales "
@@ -397,6 +402,13 @@ def main():
         b.free()
         log(f"copy probe: {nb} bytes device to device")
 
+    batched = None
+    if not args.no_batched:
+        t = time.time()
+        from indigo2suite_amd import bench_batched
+        batched = bench_batched.measure()
+        log(f"batched section in {time.time() - t:.1f}s")
+
     sel = None
     if not args.no_selector:
         t = time.time()
@@ -422,6 +434,7 @@ def main():
         "per_algo": per_algo,
         "roofline": roofline,
         "selector": sel,
+        "batched": batched,
         "cpu_baseline": cpu,
         "device": dinfo["name"],
     }

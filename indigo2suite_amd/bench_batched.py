@@ -30,18 +30,25 @@ def main(argv=None):
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--verify", type=int, default=16, help="graphs checked against the oracle (0: none)")
     a = ap.parse_args(argv)
+    print(json.dumps(measure(a.count, a.lo, a.hi, a.repeats, a.verify)), flush=True)
+
+
+def measure(count=512, lo=10, hi=18, repeats=3, verify=16) -> dict:
+    """Build the batch, pick the workgroup size by an exhaustive sweep, time
+    the captured batch against one launch sequence per graph; returns the
+    record main() prints."""
     D.init()
     t = time.time()
-    graphs = BT.make_batch_graphs(a.count, a.lo, a.hi)
+    graphs = BT.make_batch_graphs(count, lo, hi)
     n_tot = sum(g.nodes for g in graphs)
     m_tot = sum(g.edges for g in graphs)
-    print(f"generated {a.count} graphs, n={n_tot} m={m_tot} in {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+    print(f"generated {count} graphs, n={n_tot} m={m_tot} in {time.time() - t:.1f}s", file=sys.stderr, flush=True)
     db = BT.DeviceBatch(graphs)
-    tune = BT.autotune_block(db, repeats=a.repeats)
+    tune = BT.autotune_block(db, repeats=repeats)
     block = tune["best_block"]
     print("block sweep", tune, file=sys.stderr, flush=True)
     ts = []
-    for _ in range(a.repeats):
+    for _ in range(repeats):
         D.lib().igx_sync()
         t0 = time.perf_counter()
         info = db.run(7, block, use_graph=True)
@@ -50,10 +57,10 @@ def main(argv=None):
     batched_wall = float(np.median([x[0] for x in ts]))
     batched_gpu = float(np.median([x[1] for x in ts]))
     checked = 0
-    if a.verify:
+    if verify:
         import oracle as O
         outs = {k: db.results(k) for k in ("bfs", "cc", "mis")}
-        idx = np.linspace(0, a.count - 1, min(a.verify, a.count)).astype(int)
+        idx = np.linspace(0, count - 1, min(verify, count)).astype(int)
         for i in idx:
             g = graphs[i]
             assert np.array_equal(outs["bfs"][i], O.bfs_serial(g, 0)), ("bfs", i)
@@ -68,7 +75,7 @@ def main(argv=None):
     per_wall = time.perf_counter() - t0
     line = {
         "metric": "batched BFS+CC+MIS throughput vs one launch per graph",
-        "graphs": a.count, "total_nodes": n_tot, "total_edges": m_tot, "block": block,
+        "graphs": count, "total_nodes": n_tot, "total_edges": m_tot, "block": block,
         "block_sweep_seconds": tune["seconds"],
         "batched_seconds_wall": round(batched_wall, 6), "batched_seconds_events": round(batched_gpu, 6),
         "per_graph_seconds_wall": round(per_wall, 6), "per_graph_seconds_events": round(per_gpu, 6),
@@ -77,7 +84,7 @@ def main(argv=None):
         "graphs_verified": checked,
     }
     db.free()
-    print(json.dumps(line), flush=True)
+    return line
 
 
 if __name__ == "__main__":
