@@ -102,6 +102,12 @@ int igx_max_degree_vertex(const igx_graph* g);
  * Writes IGX_NUM_FEATURES doubles (order documented in graph_host.cpp). */
 #define IGX_NUM_FEATURES 16
 int igx_graph_features(const igx_graph* g, uint64_t seed, double* features);
+/* The raw 64-bit draws of the seeded counter-based stream `item` that the
+ * feature sampling uses (wedges: item 0xC1, three draws per wedge; edge
+ * locality: item 0x10C, one per edge), so that the device version of the
+ * features (igx_graph_features_device in indigo_hip.h) samples the same
+ * wedges and edges. */
+int igx_feature_draws(uint64_t seed, uint64_t item, int count, uint64_t* out);
 
 #ifdef __cplusplus
 }

@@ -200,6 +200,13 @@ int igx_batch_run(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int*
 int igx_batch_run_noalloc(const igx_batch* b, int algo_mask, int* d_bfs, int* d_cc, int* d_mis, int block,
                           igx_run_info* info);
 
+/* ------------------------------------------------------ selector features --- */
+/* No reference counterpart.  The style selector's IGX_NUM_FEATURES graph
+ * features (igx_graph_features in igx_graph.h, same order and values up to
+ * rounding of two sums) computed on the device from a device-resident CSR;
+ * *seconds (may be NULL) is the device time of the whole computation. */
+int igx_graph_features_device(const igx_graph* dev, uint64_t seed, double* features, double* seconds);
+
 /* ------------------------------------------------------ selector network ---- */
 /* No reference counterpart.  MLP in -> 64 -> 64 -> out (ReLU, FP32) trained
  * and evaluated with hand-written HIP kernels.  Parameters are laid out as

@@ -94,6 +94,7 @@ def graph_lib():
         _sig(lib, "igx_build_sp_host", ctypes.c_int, [G, c_int_p])
         _sig(lib, "igx_max_degree_vertex", ctypes.c_int, [G])
         _sig(lib, "igx_graph_features", ctypes.c_int, [G, u64, ctypes.POINTER(ctypes.c_double)])
+        _sig(lib, "igx_feature_draws", ctypes.c_int, [u64, u64, ctypes.c_int, ctypes.POINTER(u64)])
         _graph_lib = lib
     return _graph_lib
 
@@ -149,6 +150,7 @@ def hip_lib():
         _sig(lib, "igx_pr", ctypes.c_int, [G, ctypes.POINTER(ctypes.c_float), ip, dp, ip])
         _sig(lib, "igx_tc_vertex", ctypes.c_int, [ullp, ctypes.c_int, ip, ip, dp])
         _sig(lib, "igx_tc_edge", ctypes.c_int, [ullp, ctypes.c_int, ip, ip, ip, dp])
+        _sig(lib, "igx_graph_features_device", ctypes.c_int, [G, ctypes.c_uint64, dp, dp])
         _hip_lib = lib
     return _hip_lib
 

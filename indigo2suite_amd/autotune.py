@@ -6,12 +6,14 @@ default style first; a style whose first run is slower than `cap_factor` x
 the best seen so far, or than `slow_seconds`, is not repeated.  `collect` builds the selector's training data: seeded graphs from
 four families (Erdos-Renyi; R-MAT with a drawn in [0.45, 0.65] and b:c:d kept
 at 0.19:0.19:0.05; 2-D grids with long-range edges; Chung-Lu power laws with
-exponent in [2.1, 3.0]), their 16 features, and the sweep of every style.
+exponent in [2.1, 3.0]), their 16 features (computed on the device, as a
+caller of `selector.pick_style_device` gets them), and the sweep of every style.
 """
 from __future__ import annotations
 
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from typing import Dict, List, Optional
 
 import numpy as np
@@ -58,7 +60,8 @@ def build_from_spec(spec: dict) -> Graph:
 # thousands a single run takes minutes.  They are not run there; their time
 # is recorded as +inf ("too slow to measure"), which the selector learns as
 # the largest slowdown.
-UNORIENTED_TC = ("v_merge_thread", "v_merge_wave", "e_bsearch_thread")
+UNORIENTED_TC = ("v_merge_thread", "v_merge_wave", "e_bsearch_thread", "v_merge_block", "e_bsearch_wave",
+                 "e_bsearch_block")
 TC_MAX_DEGREE = 8192
 
 
@@ -91,9 +94,11 @@ def sweep(dg: device.DeviceGraph, algo: str, src: int = 0, repeats: int = 3, cap
 
 
 def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_base: int = 100000,
-            log_n_range=(12, 21), deg_range=(4, 32), repeats: int = 3, log=print, checkpoint=None) -> dict:
-    """Features and per-style times for `count` seeded graphs.  `log` gets a
-    line per graph; checkpoint(partial_dataset) is called after each graph."""
+            log_n_range=(12, 21), deg_range=(4, 32), repeats: int = 3, log=print, checkpoint=None,
+            prefetch: int = 4, checkpoint_every: int = 1) -> dict:
+    """Features (computed on the device) and per-style times for `count`
+    seeded graphs.  `log` gets a line per graph; checkpoint(partial_dataset)
+    is called every `checkpoint_every` graphs."""
     device.init()
     feats = []
     times = {a: [] for a in algos}
@@ -104,19 +109,29 @@ def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_bas
         return {"features": np.array(feats), "times": {a: np.array(v) for a, v in times.items()},
                 "styles": {a: device.styles(a) for a in algos}, "specs": specs}
 
+    # graphs are generated on host threads ahead of the sweeps (the native
+    # generators release the GIL), so the GPU does not wait for them
+    pool = ThreadPoolExecutor(max_workers=prefetch)
+    specs_all = [graph_spec(i, seed_base, log_n_range, deg_range) for i in range(count)]
+    pending = {i: pool.submit(build_from_spec, specs_all[i]) for i in range(min(prefetch, count))}
     try:
         for i in range(count):
-            spec = graph_spec(i, seed_base, log_n_range, deg_range)
-            g = build_from_spec(spec)
+            spec = specs_all[i]
+            g = pending.pop(i).result()
+            if i + prefetch < count:
+                pending[i + prefetch] = pool.submit(build_from_spec, specs_all[i + prefetch])
             src = g.max_degree_vertex()
-            feat = g.features(seed=spec["seed"])
             row = {}
             dmax = int(np.diff(g.nindex).max()) if g.nodes > 0 else 0
             with device.DeviceGraph(g) as dg:
+                feat, _ = device.features_device(dg, spec["seed"])
                 for a in algos:
                     device.set_pr_params(1e-6, 100) if a == "pr" else device.set_pr_params(1e-4, 100)
+                    ta = time.time()
                     sw = sweep(dg, a, src, repeats, max_degree=dmax)
                     row[a] = [sw[s] for s in device.styles(a)]
+                    if log and g.nodes >= (1 << 20):
+                        log(f"  graph {i + 1} {a}: {len(sw)} styles in {time.time() - ta:.1f}s")
             feats.append(feat)
             for a in algos:
                 times[a].append(row[a])
@@ -124,10 +139,11 @@ def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_bas
             specs.append(spec)
             if log:
                 log(f"graph {i + 1}/{count} {spec['family']} n={g.nodes} m={g.edges} done at {time.time() - t0:.0f}s")
-            if checkpoint:
+            if checkpoint and ((i + 1) % checkpoint_every == 0 or i + 1 == count):
                 checkpoint(dataset())
     finally:
         device.set_pr_params(1e-4, 100)
+        pool.shutdown(wait=True, cancel_futures=True)
     return dataset()
 
 

@@ -26,7 +26,7 @@ HIP_LIB = os.path.join(PKG_DIR, "libindigo_hip.so")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 
 HIP_SOURCES = ["runtime.hip", "bfs.hip", "sssp.hip", "cc.hip", "mis.hip", "pr.hip", "tc.hip",
-               "batched.hip", "mlp.hip"]
+               "batched.hip", "mlp.hip", "features.hip"]
 OFFLOAD_ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -9,6 +9,8 @@
 //   e_topo_push_*   [C-CET]:57-69 one edge per item
 //   v_data_push_*   [C-CVD]:46-136 worklist seeded with every vertex
 //   e_data_push_*   [C-CED]:48-80 edge worklist seeded with every edge
+//   v_topo_{push,pull}_{thread,wave}_determ  V_Topo_{Push,Pull}_Determ
+//                   siblings: Jacobi rounds over label / label_n buffers
 //   uf_hook_compress  union-find (no reference counterpart): parents start at
 //                   a smaller neighbour, every edge (v, u<v) hooks the larger
 //                   root under the smaller with a CAS, and a final pass points
@@ -93,6 +95,83 @@ __global__ void __launch_bounds__(kBlock) cc_vtopo_pull(const int* __restrict__ 
     if (l < l0) {
       store_relaxed(&label[v], l);
       updated = true;
+    }
+  }
+  cc_raise_flag(ctl, updated);
+}
+
+// ------------------------------------------------ deterministic (Jacobi) ----
+// CC_CUDA_V_Topo_Push_Determ_IntType_ReadModifyWrite_NonPersist_Atomic_Thread_
+// NoBoundsBug2_NoBoundsBug1.cu cc():57-77 and the V_Topo_Pull_Determ sibling
+// (..._ReadWrite_NonPersist_Atomic_Thread_... cc():58-80), Warp forms alike: a
+// round reads only `cur` and writes only `nxt`, the two trading places every
+// round, so the labels after round r are the minima over r-hop
+// neighbourhoods whatever the schedule.  Push: nxt enters a round with the
+// labels of two rounds ago (never below cur's), every vertex lowers its own
+// nxt entry to its cur label and then pushes that label to its neighbours; a
+// push counts as progress only below the neighbour's cur label, so the last
+// round changes no label and leaves both buffers equal.  Pull: every vertex
+// writes the minimum of its own and its neighbours' cur labels to nxt.
+template <int G>
+__global__ void __launch_bounds__(kBlock) cc_determ_push(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int n,
+                                                          const int* __restrict__ cur, int* __restrict__ nxt,
+                                                          Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  auto push = [&](int i, int l) {
+    const int dst = nlist[i];
+    if (load_relaxed(&nxt[dst]) > l && atomic_min_relaxed(&nxt[dst], l) > l && l < cur[dst]) updated = true;
+  };
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int l = cur[v];
+      if (load_relaxed(&nxt[v]) > l) atomic_min_relaxed(&nxt[v], l);
+      for (int i = nindex[v]; i < nindex[v + 1]; i++) push(i, l);
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int l = cur[v];
+      if (lane == 0 && load_relaxed(&nxt[v]) > l) atomic_min_relaxed(&nxt[v], l);
+      for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) push(i, l);
+    }
+  }
+  cc_raise_flag(ctl, updated);
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) cc_determ_pull(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int n,
+                                                          const int* __restrict__ cur, int* __restrict__ nxt,
+                                                          Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int l0 = cur[v];
+      int l = l0;
+      for (int i = nindex[v]; i < nindex[v + 1]; i++) l = min(l, cur[nlist[i]]);
+      nxt[v] = l;
+      updated |= l < l0;
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int l0 = cur[v];
+      long long l = l0;
+      for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) l = min(l, (long long)cur[nlist[i]]);
+      l = wave_min_ll(l);
+      if (lane == 0) {
+        nxt[v] = (int)l;
+        updated |= l < l0;
+      }
     }
   }
   cc_raise_flag(ctl, updated);
@@ -670,6 +749,41 @@ int style_afforest(const igx_graph* g, int, void* out, const void*, igx_run_info
   return IGX_OK;
 }
 
+// Both buffers start equal and end equal (see cc_determ_push), so the result
+// is in `out` whichever buffer the last round wrote.
+template <bool PULL, int G>
+int style_determ(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* l0 = (int*)out;
+  void* p1 = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)g->nodes, &p1));
+  int* l1 = (int*)p1;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(cc_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, l0, g->nodes);
+  hipLaunchKernelGGL(cc_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, l1, g->nodes);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long r) {
+        const int* cur = (r & 1) ? l1 : l0;
+        int* nxt = (r & 1) ? l0 : l1;
+        if constexpr (PULL) {
+          hipLaunchKernelGGL((cc_determ_pull<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
+                             g->nlist, g->nodes, cur, nxt, R.d_ctl);
+        } else {
+          hipLaunchKernelGGL((cc_determ_push<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
+                             g->nlist, g->nodes, cur, nxt, R.d_ctl);
+        }
+        launch_fin_flag();
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
 const StyleEntry kStyles[] = {
     {"v_topo_push_thread", style_vtopo_push<kThread, kAtomicRelaxed, true>, 0},
     {"v_topo_push_wave", style_vtopo_push<kWaveG, kAtomicRelaxed, true>, 0},
@@ -683,6 +797,10 @@ const StyleEntry kStyles[] = {
     {"e_data_push_thread", style_data_push<true>, 1},
     {"uf_hook_compress", style_uf, 0},
     {"uf_afforest", style_afforest, 0},
+    {"v_topo_push_thread_determ", style_determ<false, kThread>, 0},
+    {"v_topo_push_wave_determ", style_determ<false, kWaveG>, 0},
+    {"v_topo_pull_thread_determ", style_determ<true, kThread>, 0},
+    {"v_topo_pull_wave_determ", style_determ<true, kWaveG>, 0},
 };
 
 }  // namespace

@@ -417,6 +417,8 @@ int igx_max_degree_vertex(const igx_graph* g)
 // 13 fraction of vertices with degree >= 64
 // 14 fraction of edges owned by vertices with degree > 1024
 // 15 fraction of vertices with degree <= 4
+// Breadth-first levels from src; returns the smallest id at the largest level
+// (the device version, igx_graph_features_device, picks the same vertex).
 static int bfs_far(const igx_graph* g, int src, std::vector<int>& lvl, int* reached)
 {
   const int n = g->nodes;
@@ -429,7 +431,7 @@ static int bfs_far(const igx_graph* g, int src, std::vector<int>& lvl, int* reac
   int far = src;
   while (head < q.size()) {
     const int v = q[head++];
-    if (lvl[v] > lvl[far]) far = v;
+    if (lvl[v] > lvl[far] || (lvl[v] == lvl[far] && v < far)) far = v;  // smallest id at the largest level
     for (int i = g->nindex[v]; i < g->nindex[v + 1]; i++) {
       const int u = g->nlist[i];
       if (lvl[u] < 0) {lvl[u] = lvl[v] + 1; q.push_back(u);}
@@ -437,6 +439,14 @@ static int bfs_far(const igx_graph* g, int src, std::vector<int>& lvl, int* reac
   }
   if (reached) *reached = (int)q.size();
   return far;
+}
+
+int igx_feature_draws(uint64_t seed, uint64_t item, int count, uint64_t* out)
+{
+  if (count < 0 || (count > 0 && out == nullptr)) return IGX_ERR_ARG;
+  Stream rng(seed, item);
+  for (int k = 0; k < count; k++) out[k] = rng.next();
+  return IGX_OK;
 }
 
 int igx_graph_features(const igx_graph* g, uint64_t seed, double* f)

@@ -57,6 +57,13 @@ __device__ __forceinline__ long long wave_min_ll(long long v)
   return v;
 }
 
+__device__ __forceinline__ long long wave_max_ll(long long v)
+{
+  #pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, (long long)__shfl_xor(v, d, kWave));
+  return v;
+}
+
 __device__ __forceinline__ int wave_sum(int v)
 {
   #pragma unroll

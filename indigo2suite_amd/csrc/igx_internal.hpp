@@ -48,7 +48,7 @@ int ensure_init();
 // Cached device buffers, one per slot, grown on demand and kept until
 // igx_release_workspace() -- allocation stays outside the timed region.
 enum WsSlot {
-  WS_A = 0, WS_B, WS_C, WS_D, WS_E, WS_F, WS_G, WS_H, WS_SP, WS_SCRATCH, WS_BLOCKS, WS_COUNT
+  WS_A = 0, WS_B, WS_C, WS_D, WS_E, WS_F, WS_G, WS_H, WS_SP, WS_SCRATCH, WS_BLOCKS, WS_FEAT, WS_COUNT
 };
 int ws_get(int slot, size_t bytes, void** ptr);
 

@@ -11,6 +11,7 @@
 //   v_topo_pull_*  V_Topo_Pull siblings (exclude self when a neighbour is in)
 //   v_data_pull_*  V_Data_Pull siblings: worklist of undecided vertices
 //   e_topo_thread  [C-MET]:57-127 / [O-MISe]:52-94 edge pass + vertex pass
+//   e_data_thread  [C-MED]:45-213 worklist of edges, edge pass + vertex pass
 //   v_resume_pull  (no reference counterpart) pull over the outranking
 //                  neighbours only, resuming where the last round stopped
 #include "igx_internal.hpp"
@@ -208,6 +209,101 @@ __global__ void __launch_bounds__(kBlock) mis_vertex_pass(int n, int* __restrict
     }
   }
   mis_raise_flag(ctl, pending);
+}
+
+// ------------------------------------------------------ edge, data-driven ----
+// [C-MED] init():45-62, mis():64-91, mis_vertex_pass():93-126,
+// mis_last_pass():128-137: a worklist of edges, each
+// undirected edge once (src < dst), shrinks as both endpoints get decided.
+// Per round an edge pass excludes the other endpoint of an included vertex and
+// marks the lower-ranked endpoint of an undecided pair as lost; a vertex pass
+// over the same edges includes every undecided endpoint that did not lose and
+// keeps the edges that still have an undecided endpoint.  A vertex left
+// undecided at the end has no edge to anyone else and joins.
+//
+// `lost` holds the number of the round in which a vertex lost (rounds count
+// from 1), so it never needs clearing; the reference clears a 0/1 flag array
+// every round.  Statuses use one array, as in the topology-driven edge style:
+// an edge pass sees every inclusion of earlier rounds (the kernel boundary
+// publishes them) and, within the pass, a stale "undecided" only adds a lost
+// mark, which delays a vertex by a round and never decides one wrongly.  The
+// worklist never holds an edge twice, so the reference's per-edge time stamps
+// that suppress duplicate pushes are not needed.
+__global__ void __launch_bounds__(kBlock) mis_edata_seed(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                          int m, int* __restrict__ wl, Ctl* ctl)
+{
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < m; base += stride) {  // uniform per wave
+    const int e = base + threadIdx.x;
+    const bool keep = e < m && sp[e] < nlist[e];
+    const int slot = wave_push_slot(&ctl->wl_count[0], keep);
+    if (keep) wl[slot] = e;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) mis_edata_edge(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                          int* __restrict__ status, int* __restrict__ lost,
+                                                          const int* __restrict__ wl0, const int* __restrict__ wl1,
+                                                          Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int p = ctl->iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  const int stamp = ctl->iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+    const int e = wl_in[idx];
+    const int src = sp[e];
+    const int dst = nlist[e];
+    const int ss = status[src];
+    const int sd = status[dst];
+    if (ss == kIncluded) {
+      status[dst] = kExcluded;
+    } else if (sd == kIncluded) {
+      status[src] = kExcluded;
+    } else if (ss == kUndecided && sd == kUndecided) {
+      const bool dst_wins = outranks(mis_priority(dst), dst, mis_priority(src), src);
+      lost[dst_wins ? src : dst] = stamp;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) mis_edata_vertex(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                            int* __restrict__ status, const int* __restrict__ lost,
+                                                            int* __restrict__ wl0, int* __restrict__ wl1, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int p = ctl->iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int stamp = ctl->iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < count; base += stride) {  // uniform per wave
+    const int idx = base + threadIdx.x;
+    bool keep = false;
+    int e = 0;
+    if (idx < count) {
+      e = wl_in[idx];
+      const int src = sp[e];
+      const int dst = nlist[e];
+      const int ss = status[src];
+      const int sd = status[dst];
+      if (ss == kUndecided && lost[src] != stamp) status[src] = kIncluded;
+      if (sd == kUndecided && lost[dst] != stamp) status[dst] = kIncluded;
+      keep = ss == kUndecided || sd == kUndecided;
+    }
+    const int slot = wave_push_slot(out_count, keep);
+    if (keep) wl_out[slot] = e;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) mis_edata_last(int* __restrict__ status, int n)
+{
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n && status[v] == kUndecided) status[v] = kIncluded;
 }
 
 // ------------------------------------------------------- resumable pull ----
@@ -438,6 +534,41 @@ int style_etopo(const igx_graph* g, int, void* out, const void* aux, igx_run_inf
   });
 }
 
+int style_edata(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* status = (int*)out;
+  const int* sp = (const int*)aux;
+  const size_t wl_items = (size_t)(g->edges > 0 ? g->edges : 1);
+  void *pl = nullptr, *p0 = nullptr, *p1 = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)g->nodes, &pl));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * wl_items, &p0));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * wl_items, &p1));
+  int* lost = (int*)pl;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, status, g->nodes);
+  hipLaunchKernelGGL(mis_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, lost, g->nodes);
+  hipLaunchKernelGGL(mis_edata_seed, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, sp, g->edges,
+                     (int*)p0, R.d_ctl);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        hipLaunchKernelGGL(mis_edata_edge, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, sp, status,
+                           lost, (const int*)p0, (const int*)p1, R.d_ctl);
+        hipLaunchKernelGGL(mis_edata_vertex, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, sp,
+                           status, (const int*)lost, (int*)p0, (int*)p1, R.d_ctl);
+        launch_fin_worklist();
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  hipLaunchKernelGGL(mis_edata_last, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, status,
+                     g->nodes);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
 const StyleEntry kStyles[] = {
     {"v_topo_push_thread", style_vtopo_push<false>, 0},
     {"v_topo_push_wave", style_vtopo_push<true>, 0},
@@ -445,6 +576,7 @@ const StyleEntry kStyles[] = {
     {"v_data_pull_thread", style_vdata_pull, 0},
     {"e_topo_thread", style_etopo, 1},
     {"v_resume_pull", style_resume_pull, 0},
+    {"e_data_thread", style_edata, 1},
 };
 
 }  // namespace

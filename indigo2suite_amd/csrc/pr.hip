@@ -22,6 +22,9 @@
 //                       into chunks of at most kChunk edges (one wave each)
 //                       whose compensated partial sums are merged in chunk
 //                       order
+//   push_fixed_{thread,wave,block}  [C-PRpush] Push siblings: contributions
+//                       pushed to the neighbours' totals with 64-bit
+//                       fixed-point atomic adds (exact, order-independent)
 // The contribution of the next iteration (score / degree) is written by the
 // same kernel that produces the score, so one pass per iteration remains.
 #include "igx_internal.hpp"
@@ -316,6 +319,82 @@ __global__ void __launch_bounds__(kBlock) pr_bal_big(const int* __restrict__ nin
   round_partial_ll(ctl, 1, err, s_red);
 }
 
+// ----------------------------------------------------------------- push ----
+// [C-PRpush] contrib():41-48, push():50-63, compute():65-77: every vertex adds
+// its contribution score / degree to each neighbour's incoming total, then
+// every vertex turns its total into its new score.  The reference adds floats
+// atomically, so its totals depend on the order in which the atomics land.
+// Here each contribution is rounded once to a multiple of 2^-60 (a float
+// contribution of at least 2^-37 already is one, so typical contributions are
+// not rounded at all) and added as a 64-bit integer: integer addition is
+// associative, so every total, and with it the iteration count, is the same
+// whatever the schedule, and it is the exact sum of the rounded contributions.
+// A total is at most the sum of all scores (at most 1 = 2^60 units), far from
+// the int64 limit.  The compute pass reads each total once, clears it for the
+// next iteration, and writes the score, the next contribution and the
+// residual exactly like the pull styles.
+constexpr double kFixScale = 1152921504606846976.0;  // 2^60: fixed-point unit of the totals
+
+__device__ __forceinline__ unsigned long long pr_fixed(float c)
+{
+  return (unsigned long long)((double)c * kFixScale + 0.5);
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) pr_push(const int* __restrict__ nindex, const int* __restrict__ nlist,
+                                                   int n, const float* __restrict__ contrib,
+                                                   unsigned long long* __restrict__ incoming, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int beg = nindex[v], end = nindex[v + 1];
+      if (beg == end) continue;  // a vertex of degree 0 has an infinite contribution and no edge
+      const unsigned long long q = pr_fixed(contrib[v]);
+      for (int i = beg; i < end; i++) atomicAdd(&incoming[nlist[i]], q);
+    }
+  } else if constexpr (G == kWaveG) {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int beg = nindex[v], end = nindex[v + 1];
+      if (beg == end) continue;
+      const unsigned long long q = pr_fixed(contrib[v]);
+      for (int i = beg + lane; i < end; i += kWave) atomicAdd(&incoming[nlist[i]], q);
+    }
+  } else {
+    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+      const int beg = nindex[v], end = nindex[v + 1];
+      if (beg == end) continue;
+      const unsigned long long q = pr_fixed(contrib[v]);
+      for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) atomicAdd(&incoming[nlist[i]], q);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) pr_push_compute(const int* __restrict__ nindex, int n, float base,
+                                                           float* __restrict__ score, float* __restrict__ contrib,
+                                                           unsigned long long* __restrict__ incoming, Ctl* ctl)
+{
+  __shared__ long long s_red[kWavesPerBlock];
+  if (round_done(ctl)) return;
+  long long err = 0;
+  const int stride = gridDim.x * kBlock;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+    const float total = (float)((double)incoming[v] / kFixScale);
+    incoming[v] = 0;
+    err += pr_finish(v, Pair{total, 0.0f}, base, nindex, score, contrib);
+  }
+  round_partial_ll(ctl, 0, err, s_red);
+}
+
+__global__ void __launch_bounds__(kBlock) pr_zero_ull(unsigned long long* __restrict__ p, int n)
+{
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // ------------------------------------------------------------ host side ----
 int pr_buffers(const igx_graph* g, float** c0, float** c1)
 {
@@ -402,11 +481,52 @@ int style_balanced(const igx_graph* g, int, void* out, const void*, igx_run_info
   return rc == IGX_ERR_LIMIT ? IGX_OK : rc;
 }
 
+// One contribution buffer suffices: the push pass reads it and the compute
+// pass, a separate launch, rewrites it.
+template <int G>
+int style_push(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  float* score = (float*)out;
+  const int n = g->nodes;
+  void *pc = nullptr, *pin = nullptr;
+  IGX_TRY(ws_get(WS_F, sizeof(float) * (size_t)n, &pc));
+  IGX_TRY(ws_get(WS_H, sizeof(unsigned long long) * (size_t)n, &pin));
+  float* contrib = (float*)pc;
+  unsigned long long* incoming = (unsigned long long*)pin;
+  const PrParams P = pr_params();
+  const float base = (1.0f - kDamp) / (float)n;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(n));
+  hipLaunchKernelGGL(pr_init, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, R.stream, g->nindex, n, score, contrib);
+  hipLaunchKernelGGL(pr_zero_ull, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, R.stream, incoming, n);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        ktimer_begin("pr_push");
+        hipLaunchKernelGGL((pr_push<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, n,
+                           contrib, incoming, R.d_ctl);
+        ktimer_end("pr_push");
+        hipLaunchKernelGGL(pr_push_compute, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, n, base,
+                           score, contrib, incoming, R.d_ctl);
+        launch_finalize(R.persist_blocks, 0x1, FinPr{P.eps, P.max_iter});
+      },
+      P.max_iter, &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  info->residual = R.h_ctl->res_last;
+  return rc == IGX_ERR_LIMIT ? IGX_OK : rc;
+}
+
 const StyleEntry kStyles[] = {
     {"pull_thread_kahan", style_pull<kThread>, 0},
     {"pull_wave_kahan", style_pull<kWaveG>, 0},
     {"pull_block_kahan", style_pull<kBlockG>, 0},
     {"pull_balanced_kahan", style_balanced, 0},
+    {"push_fixed_thread", style_push<kThread>, 0},
+    {"push_fixed_wave", style_push<kWaveG>, 0},
+    {"push_fixed_block", style_push<kBlockG>, 0},
 };
 
 }  // namespace

@@ -12,6 +12,8 @@
 //                   itself is re-queued to repair a lost update
 //   v_data_push_balanced   (no reference counterpart) vertex worklist of edge
 //                   chunks expanded by wave-level merge path
+//   v_topo_{push,pull}_{thread,wave}_determ  V_Topo_{Push,Pull}_Determ
+//                   siblings: Jacobi rounds over two distance buffers
 #include "igx_internal.hpp"
 #include "igx_frontier.hpp"
 
@@ -122,6 +124,101 @@ __global__ void __launch_bounds__(kBlock) sssp_etopo_push(const int* __restrict_
       const int nd = s + ew[e];
       const int dst = nlist[e];
       if (dist[dst] > nd && atomic_min_relaxed(&dist[dst], nd) > nd) updated = true;
+    }
+  }
+  sssp_raise_flag(ctl, updated);
+}
+
+// ------------------------------------------------ deterministic (Jacobi) ----
+// SSSP_CUDA_V_Topo_Push_Determ_IntType_ReadModifyWrite_NonPersist_Atomic_
+// Thread_NoBoundsBug2_NoBoundsBug1.cu sssp():57-80 and the V_Topo_Pull_Determ
+// sibling (..._ReadWrite_NonPersist_Atomic_Thread_... sssp():57-79), Warp
+// forms alike.  A round reads only `cur` and writes only `nxt`, and the two
+// trade places every round, so the distances after round r are the r-hop
+// Bellman-Ford values whatever the schedule, and the round count is fixed by
+// the graph and the source.
+//
+// Push: nxt enters a round holding the values of two rounds ago, never below
+// cur's (every entry only decreases), so a reached vertex first lowers its own
+// nxt entry to its cur value and then relaxes its edges into nxt.  A
+// relaxation counts as progress only if it goes below the destination's cur
+// value; otherwise it merely caught nxt up.  The last round therefore changes
+// no value of cur, and leaves nxt equal to cur: both buffers hold the result.
+// Pull: every vertex writes min(cur[v], min over edges of cur[u] + w) to nxt,
+// so again both buffers are equal after the last round.
+template <int G>
+__global__ void __launch_bounds__(kBlock) sssp_determ_push(const int* __restrict__ nindex,
+                                                            const int* __restrict__ nlist,
+                                                            const int* __restrict__ ew, int n,
+                                                            const int* __restrict__ cur, int* __restrict__ nxt,
+                                                            Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  auto relax = [&](int i, int s) {
+    const int nd = s + ew[i];
+    const int dst = nlist[i];
+    if (load_relaxed(&nxt[dst]) > nd && atomic_min_relaxed(&nxt[dst], nd) > nd && nd < cur[dst]) updated = true;
+  };
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int s = cur[v];
+      if (s != kInf) {
+        if (load_relaxed(&nxt[v]) > s) atomic_min_relaxed(&nxt[v], s);
+        for (int i = nindex[v]; i < nindex[v + 1]; i++) relax(i, s);
+      }
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int s = cur[v];
+      if (s != kInf) {
+        if (lane == 0 && load_relaxed(&nxt[v]) > s) atomic_min_relaxed(&nxt[v], s);
+        for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) relax(i, s);
+      }
+    }
+  }
+  sssp_raise_flag(ctl, updated);
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) sssp_determ_pull(const int* __restrict__ nindex,
+                                                            const int* __restrict__ nlist,
+                                                            const int* __restrict__ ew, int n,
+                                                            const int* __restrict__ cur, int* __restrict__ nxt,
+                                                            Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  bool updated = false;
+  if constexpr (G == kThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+      const int d0 = cur[v];
+      int d = d0;
+      for (int i = nindex[v]; i < nindex[v + 1]; i++) {
+        const int s = cur[nlist[i]];
+        if (s != kInf && s + ew[i] < d) d = s + ew[i];
+      }
+      nxt[v] = d;
+      updated |= d < d0;
+    }
+  } else {
+    const int waves = gridDim.x * kWavesPerBlock;
+    const int lane = lane_id();
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
+      const int d0 = cur[v];
+      long long d = d0;
+      for (int i = nindex[v] + lane; i < nindex[v + 1]; i += kWave) {
+        const int s = cur[nlist[i]];
+        if (s != kInf && s + ew[i] < d) d = s + ew[i];
+      }
+      d = wave_min_ll(d);
+      if (lane == 0) {
+        nxt[v] = (int)d;
+        updated |= d < d0;
+      }
     }
   }
   sssp_raise_flag(ctl, updated);
@@ -788,6 +885,41 @@ int style_vtopo_pull(const igx_graph* g, int src, void* out, const void*, igx_ru
   }, false);
 }
 
+// Both buffers start equal and end equal (see sssp_determ_push), so the
+// result is in `out` whichever buffer the last round wrote.
+template <bool PULL, int G>
+int style_determ(const igx_graph* g, int src, void* out, const void*, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* d0 = (int*)out;
+  void* p1 = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)g->nodes, &p1));
+  int* d1 = (int*)p1;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, d0, g->nodes);
+  hipLaunchKernelGGL(sssp_init, dim3(blocks_for(g->nodes, kBlock)), dim3(kBlock), 0, R.stream, src, d1, g->nodes);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long r) {
+        const int* cur = (r & 1) ? d1 : d0;
+        int* nxt = (r & 1) ? d0 : d1;
+        if constexpr (PULL) {
+          hipLaunchKernelGGL((sssp_determ_pull<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
+                             g->nlist, g->eweight, g->nodes, cur, nxt, R.d_ctl);
+        } else {
+          hipLaunchKernelGGL((sssp_determ_push<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex,
+                             g->nlist, g->eweight, g->nodes, cur, nxt, R.d_ctl);
+        }
+        launch_fin_flag();
+      },
+      sssp_round_cap(g), &rounds, &launches);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
 int style_etopo_push(const igx_graph* g, int src, void* out, const void* aux, igx_run_info* info)
 {
   Runtime& R = rt();
@@ -930,6 +1062,10 @@ const StyleEntry kStyles[] = {
     {"e_data_push_rw_thread", style_edata_push<true>, 1},
     {"v_data_push_balanced", style_balanced, 0},
     {"v_do_balanced", style_do, 0},
+    {"v_topo_push_thread_determ", style_determ<false, kThread>, 0},
+    {"v_topo_push_wave_determ", style_determ<false, kWaveG>, 0},
+    {"v_topo_pull_thread_determ", style_determ<true, kThread>, 0},
+    {"v_topo_pull_wave_determ", style_determ<true, kWaveG>, 0},
 };
 
 }  // namespace

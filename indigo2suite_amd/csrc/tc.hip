@@ -7,17 +7,24 @@
 //                    (tc_vertex_cuda.h:44-63): for v and each neighbour u > v,
 //                    merge-count the neighbours above u common to both lists
 //   v_merge_wave     TC_V_Warp siblings: one wave per vertex, lanes take the u
+//   v_merge_block    TC_V_Block_NonPersist_Reduction_IntType_Atomic.cu:43-82:
+//                    one workgroup per vertex, threads take the u, wave
+//                    shuffles + LDS reduce
 //   e_bsearch_thread [C-TE]:42-61 with d_find (tc_edge_cuda.h:116-127): for each
 //                    edge src > dst, look every u < dst of N(dst) up in N(src)
 //                    (exclusive upper bound here; the reference's inclusive
 //                    bound reads one past the list)
+//   e_bsearch_wave   TC_E_Warp_NonPersist_Reduction_IntType_Atomic.cu:43-82:
+//                    one wave per edge, lanes take the u of N(dst)
+//   e_bsearch_block  TC_E_Block siblings: one workgroup per edge
 //   oriented_lds     (no reference counterpart) degree-ordered orientation:
 //                    each edge points from lower to higher (degree, id) rank,
 //                    so hub lists become short; one wave per vertex keeps its
 //                    out-list as a hash set in LDS and probes the out-lists of
 //                    its out-neighbours into it, 64 elements per step
 // Counts are reduced per wave and added to one uint64 with a single atomic
-// per wave (the reference's GlobalAdd/BlockAdd/Reduction axis).
+// per wave, or per workgroup in the Block forms (the reference's
+// GlobalAdd/BlockAdd/Reduction axis).
 #include "igx_internal.hpp"
 #include "igx_frontier.hpp"
 #include "igx_scan.hpp"
@@ -55,6 +62,25 @@ __device__ __forceinline__ void wave_add_count(unsigned long long* out, unsigned
   if (lane_id() == 0 && c != 0) atomicAdd(out, c);
 }
 
+// the reference's Reduction form: wave shuffles, then one partial per wave in
+// LDS, summed by the first wave, one atomic per workgroup
+__device__ __forceinline__ void block_add_count(unsigned long long* out, unsigned long long c,
+                                                unsigned long long* s_red)
+{
+  #pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) s_red[w] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) t += s_red[k];
+    if (t != 0) atomicAdd(out, t);
+  }
+}
+
+enum TcGran { kTcThread = 0, kTcWave = 1, kTcBlock = 2 };
+
 // first index in [beg, end) whose value is > key (sorted list)
 __device__ __forceinline__ int upper_pos(const int* __restrict__ nlist, int beg, int end, int key)
 {
@@ -83,70 +109,99 @@ __device__ __forceinline__ int merge_common(int b1, int e1, int b2, int e2, cons
   return c;
 }
 
-template <bool WAVE>
+// one vertex per thread (Thread), per wave (Warp: lanes take the u) or per
+// workgroup (Block: threads take the u, the Block_Reduction form)
+template <int G>
 __global__ void __launch_bounds__(kBlock) tc_vertex_merge(const int* __restrict__ nindex,
                                                            const int* __restrict__ nlist, int n,
                                                            unsigned long long* __restrict__ count)
 {
+  __shared__ unsigned long long s_red[kWavesPerBlock];
   unsigned long long c = 0;
-  if constexpr (!WAVE) {
+  auto one_vertex = [&](int v, int me, int width) {
+    const int beg1 = nindex[v], end1 = nindex[v + 1];
+    const int start1 = upper_pos(nlist, beg1, end1, v);
+    for (int j = start1 + me; j < end1; j += width) {
+      const int u = nlist[j];
+      const int beg2 = nindex[u], end2 = nindex[u + 1];
+      const int start2 = upper_pos(nlist, beg2, end2, u);
+      c += (unsigned long long)merge_common(j + 1, end1, start2, end2, nlist);
+    }
+  };
+  if constexpr (G == kTcThread) {
     const int stride = gridDim.x * kBlock;
-    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
-      const int beg1 = nindex[v], end1 = nindex[v + 1];
-      const int start1 = upper_pos(nlist, beg1, end1, v);
-      for (int j = start1; j < end1; j++) {
-        const int u = nlist[j];
-        const int beg2 = nindex[u], end2 = nindex[u + 1];
-        const int start2 = upper_pos(nlist, beg2, end2, u);
-        c += (unsigned long long)merge_common(j + 1, end1, start2, end2, nlist);
-      }
-    }
-  } else {
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) one_vertex(v, 0, 1);
+    wave_add_count(count, c);
+  } else if constexpr (G == kTcWave) {
     const int waves = gridDim.x * kWavesPerBlock;
-    const int lane = lane_id();
-    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) {
-      const int beg1 = nindex[v], end1 = nindex[v + 1];
-      const int start1 = upper_pos(nlist, beg1, end1, v);
-      for (int j = start1 + lane; j < end1; j += kWave) {
-        const int u = nlist[j];
-        const int beg2 = nindex[u], end2 = nindex[u + 1];
-        const int start2 = upper_pos(nlist, beg2, end2, u);
-        c += (unsigned long long)merge_common(j + 1, end1, start2, end2, nlist);
-      }
-    }
+    for (int v = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < n; v += waves) one_vertex(v, lane_id(), kWave);
+    wave_add_count(count, c);
+  } else {
+    for (int v = blockIdx.x; v < n; v += gridDim.x) one_vertex(v, (int)threadIdx.x, kBlock);
+    block_add_count(count, c, s_red);
   }
-  wave_add_count(count, c);
 }
 
+// is u in the sorted range [lo, hi)?
+__device__ __forceinline__ bool sorted_find(const int* __restrict__ nlist, int lo, int hi, int u)
+{
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int x = nlist[mid];
+    if (x == u) return true;
+    if (x < u) lo = mid + 1;
+    else hi = mid;
+  }
+  return false;
+}
+
+// one edge per thread (Thread), per wave (Warp: lanes take the u of N(dst))
+// or per workgroup (Block)
+template <int G>
 __global__ void __launch_bounds__(kBlock) tc_edge_bsearch(const int* __restrict__ nindex,
                                                            const int* __restrict__ nlist, const int* __restrict__ sp,
                                                            int m, unsigned long long* __restrict__ count)
 {
+  __shared__ unsigned long long s_red[kWavesPerBlock];
   unsigned long long c = 0;
-  const int stride = gridDim.x * kBlock;
-  for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
-    const int src = sp[e];
-    const int dst = nlist[e];
-    if (src > dst) {
-      const int beg2 = nindex[src], end2 = nindex[src + 1];
-      for (int i = nindex[dst]; i < nindex[dst + 1]; i++) {
-        const int u = nlist[i];
-        if (u >= dst) break;
-        int lo = beg2, hi = end2;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          const int x = nlist[mid];
-          if (x == u) {
-            c++;
-            break;
-          }
-          if (x < u) lo = mid + 1;
-          else hi = mid;
+  if constexpr (G == kTcThread) {
+    const int stride = gridDim.x * kBlock;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+      const int src = sp[e];
+      const int dst = nlist[e];
+      if (src > dst) {
+        const int beg2 = nindex[src], end2 = nindex[src + 1];
+        for (int i = nindex[dst]; i < nindex[dst + 1]; i++) {
+          const int u = nlist[i];
+          if (u >= dst) break;
+          c += sorted_find(nlist, beg2, end2, u) ? 1 : 0;
         }
       }
     }
+    wave_add_count(count, c);
+  } else {
+    // Warp / Block: the items of one edge are the u of N(dst) below dst,
+    // taken `width` at a time; the list is sorted, so once a lane meets
+    // u >= dst every later item of the edge is past the bound too
+    const int width = (G == kTcWave) ? kWave : kBlock;
+    const int me = (G == kTcWave) ? lane_id() : (int)threadIdx.x;
+    const int first = (G == kTcWave) ? blockIdx.x * kWavesPerBlock + (int)threadIdx.x / kWave : (int)blockIdx.x;
+    const int step = (G == kTcWave) ? gridDim.x * kWavesPerBlock : gridDim.x;
+    for (int e = first; e < m; e += step) {
+      const int src = sp[e];
+      const int dst = nlist[e];
+      if (src <= dst) continue;
+      const int beg2 = nindex[src], end2 = nindex[src + 1];
+      const int end1 = nindex[dst + 1];
+      for (int i = nindex[dst] + me; i < end1; i += width) {
+        const int u = nlist[i];
+        if (u >= dst) break;
+        c += sorted_find(nlist, beg2, end2, u) ? 1 : 0;
+      }
+    }
+    if constexpr (G == kTcWave) wave_add_count(count, c);
+    else block_add_count(count, c, s_red);
   }
-  wave_add_count(count, c);
 }
 
 // ------------------------------------------------------------- oriented ----
@@ -342,7 +397,7 @@ int check_sorted(const igx_graph* g, const int* sp)
   return h ? IGX_ERR_UNSORTED : IGX_OK;
 }
 
-template <bool WAVE>
+template <int G>
 int style_vertex(const igx_graph* g, int, void* out, const void*, igx_run_info* info)
 {
   Runtime& R = rt();
@@ -350,7 +405,7 @@ int style_vertex(const igx_graph* g, int, void* out, const void*, igx_run_info* 
   IGX_TRY(check_sorted(g, nullptr));
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
-  hipLaunchKernelGGL((tc_vertex_merge<WAVE>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
+  hipLaunchKernelGGL((tc_vertex_merge<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist,
                      g->nodes, count);
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
@@ -360,6 +415,7 @@ int style_vertex(const igx_graph* g, int, void* out, const void*, igx_run_info* 
   return IGX_OK;
 }
 
+template <int G>
 int style_edge(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
 {
   Runtime& R = rt();
@@ -368,7 +424,7 @@ int style_edge(const igx_graph* g, int, void* out, const void* aux, igx_run_info
   IGX_TRY(check_sorted(g, sp));
   IGX_TRY(timer_start());
   hipLaunchKernelGGL(tc_zero, dim3(1), dim3(64), 0, R.stream, count, (int*)nullptr);
-  hipLaunchKernelGGL(tc_edge_bsearch, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, sp,
+  hipLaunchKernelGGL((tc_edge_bsearch<G>), dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nindex, g->nlist, sp,
                      g->edges, count);
   IGX_HIP(hipGetLastError());
   IGX_TRY(timer_stop(&info->seconds));
@@ -426,10 +482,13 @@ int style_oriented(const igx_graph* g, int, void* out, const void*, igx_run_info
 }
 
 const StyleEntry kStyles[] = {
-    {"v_merge_thread", style_vertex<false>, 0},
-    {"v_merge_wave", style_vertex<true>, 0},
-    {"e_bsearch_thread", style_edge, 1},
+    {"v_merge_thread", style_vertex<kTcThread>, 0},
+    {"v_merge_wave", style_vertex<kTcWave>, 0},
+    {"e_bsearch_thread", style_edge<kTcThread>, 1},
     {"oriented_lds", style_oriented, 0},
+    {"v_merge_block", style_vertex<kTcBlock>, 0},
+    {"e_bsearch_wave", style_edge<kTcWave>, 1},
+    {"e_bsearch_block", style_edge<kTcBlock>, 1},
 };
 
 }  // namespace

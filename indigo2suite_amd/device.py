@@ -121,6 +121,17 @@ class DeviceGraph:
             pass
 
 
+def features_device(dg: DeviceGraph, seed: int = 1):
+    """The selector's 16 graph features computed on the device from the
+    uploaded CSR (igx_graph_features_device); returns (features, seconds)."""
+    from .graph import NUM_FEATURES
+    out = (ctypes.c_double * NUM_FEATURES)()
+    sec = ctypes.c_double(0.0)
+    check(lib().igx_graph_features_device(ctypes.byref(dg.c), seed, out, ctypes.byref(sec)),
+          "igx_graph_features_device")
+    return np.array(out[:], dtype=np.float64), sec.value
+
+
 @dataclass
 class RunResult:
     out: Union[np.ndarray, int]

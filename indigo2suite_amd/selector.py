@@ -200,9 +200,23 @@ def pick_style(graph, algo: str, model_dir: Optional[str] = None) -> str:
     ("bfs", "sssp", "cc", "mis", "pr" or "tc"), from the models shipped in
     selector_models/ unless model_dir names another directory.  Needs the GPU
     library (the MLP forward pass runs on the device)."""
+    sel = _model(algo, model_dir)
+    return sel.styles[sel.pick(graph.features())[0]]
+
+
+def _model(algo: str, model_dir: Optional[str] = None) -> "Selector":
     d = model_dir or MODEL_DIR
     key = (d, algo)
     if key not in _loaded:
         _loaded[key] = Selector.load(os.path.join(d, f"selector_{algo}.npz"))
-    sel = _loaded[key]
-    return sel.styles[sel.pick(graph.features())[0]]
+    return _loaded[key]
+
+
+def pick_style_device(dg, algo: str, model_dir: Optional[str] = None, seed: int = 1):
+    """Like pick_style, for a graph already on the device (a DeviceGraph):
+    the features are computed there (igx_graph_features_device), so the CSR
+    is not copied back.  Returns (style name, feature seconds)."""
+    from . import device
+    f, secs = device.features_device(dg, seed)
+    sel = _model(algo, model_dir)
+    return sel.styles[sel.pick(f)[0]], secs

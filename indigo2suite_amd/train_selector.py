@@ -33,7 +33,8 @@ def cmd_collect(a):
     data = autotune.collect(a.count, ALGOS, seed_base=a.seed_base, log_n_range=tuple(a.log_n),
                             deg_range=tuple(a.deg), repeats=a.repeats,
                             log=lambda s: print(s, flush=True),
-                            checkpoint=lambda d: autotune.save_dataset(a.out, d))
+                            checkpoint=lambda d: autotune.save_dataset(a.out, d),
+                            checkpoint_every=a.checkpoint_every)
     autotune.save_dataset(a.out, data)
     print(f"wrote {a.out}: {a.count} graphs in {time.time() - t0:.0f}s", flush=True)
 
@@ -96,6 +97,7 @@ def main(argv=None):
     c.add_argument("--deg", type=float, nargs=2, default=(4, 32))
     c.add_argument("--repeats", type=int, default=3)
     c.add_argument("--out", required=True)
+    c.add_argument("--checkpoint-every", type=int, default=25, help="save the partial dataset every N graphs")
     f = sub.add_parser("fit")
     f.add_argument("--train", required=True, nargs="+")
     f.add_argument("--out-dir", required=True)

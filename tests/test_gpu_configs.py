@@ -1,10 +1,10 @@
 """GPU parity at the full sizes of the measured workloads, against the oracle.
 
 * BFS on Erdos-Renyi n = 16384, average degree 8, from source 0: every style.
-* SSSP and CC on R-MAT scale 22 (n = 4,194,304, edge factor 16, seed 1,
+* SSSP, CC and MIS on R-MAT scale 22 (n = 4,194,304, edge factor 16, seed 1,
   weights U[1,255] equal in both directions), SSSP from the highest-degree
   vertex: every style, plus the reference-signature entries with host sp[].
-  BFS and MIS (default styles) on the same graph.
+  BFS (default style) on the same graph.
 * PageRank (d = 0.85, L1 < 1e-6 or 100 iterations) and triangle counting on
   R-MAT scale 20 and on the 2048 x 2048 grid with 5% long-range edges:
   PageRank within 1e-5 relative of the FP64 oracle at the same iteration
@@ -78,7 +78,7 @@ def _ref(refs, key, fn):
     return refs[key]
 
 
-@pytest.mark.parametrize("si", range(12))
+@pytest.mark.parametrize("si", range(16))
 def test_rmat22_sssp_style(gpu, rmat22, si):
     g, dg, src, refs = rmat22
     names = gpu.styles("sssp")
@@ -99,7 +99,7 @@ def test_rmat22_sssp_reference_entries(gpu, rmat22):
     assert np.array_equal(dist, ref) and t > 0
 
 
-@pytest.mark.parametrize("si", range(12))
+@pytest.mark.parametrize("si", range(16))
 def test_rmat22_cc_style(gpu, rmat22, si):
     g, dg, _, refs = rmat22
     names = gpu.styles("cc")
@@ -129,6 +129,14 @@ def test_rmat22_bfs_and_mis_defaults(gpu, rmat22):
     st = gpu.run("mis", dg, None).out
     assert np.array_equal(st, O.mis_greedy(g).astype(np.int32))
     assert O.mis_verify(g, st) == 0
+
+
+def test_rmat22_mis_all_styles(gpu, rmat22):
+    g, dg, _, refs = rmat22
+    ref = _ref(refs, "mis", lambda: O.mis_greedy(g).astype(np.int32))
+    for st in gpu.styles("mis"):
+        _say("rmat22 mis", st)
+        assert np.array_equal(gpu.run("mis", dg, st).out, ref), st
 
 
 # ------------------------------------------------ R-MAT 20 and grid: PR, TC

@@ -139,3 +139,47 @@ def test_features_shape_and_values():
     assert f[0] == 12.0
     grid = G.grid2d(32, 0.0, seed=1).features()
     assert grid[7] == 0.0 and abs(grid[8] - np.log2(1 + 62)) < 1e-9  # grid diameter 2*(32-1)
+
+
+def _star(n):
+    return G.from_edges(n, np.zeros(n - 1, np.int32), np.arange(1, n, dtype=np.int32), name=f"star{n}")
+
+
+def _path(n):
+    return G.from_edges(n, np.arange(n - 1, dtype=np.int32), np.arange(1, n, dtype=np.int32), name=f"path{n}")
+
+
+def _complete(n):
+    s, d = np.triu_indices(n, 1)
+    return G.from_edges(n, s.astype(np.int32), d.astype(np.int32), name=f"k{n}")
+
+
+def test_features_closed_forms():
+    """Features of graphs whose statistics have closed forms: a star (Gini
+    (n-2)/(2n), diameter 2, no triangles, the top 1% = centre + n/100 - 1
+    leaves), a path (diameter n-1, every edge joins ids one apart), a
+    complete graph (Gini 0, clustering 1, diameter 1) and isolated vertices."""
+    n = 1000
+    f = _star(n).features()
+    assert f[2] == 2.0 * (n - 1) / n and f[4] == np.log2(n)
+    assert abs(f[5] - (n - 2) / (2.0 * n)) < 1e-12
+    assert f[7] == 0.0 and f[8] == np.log2(3) and f[9] == 0.0 and f[10] == 1.0
+    assert abs(f[11] - (n - 1 + n // 100 - 1) / (2.0 * (n - 1))) < 1e-12
+    assert f[13] == 1.0 / n and f[14] == 0.0 and f[15] == (n - 1) / n
+    p = _path(n).features()
+    assert p[8] == np.log2(n) and p[9] == 0.0 and abs(p[12] - 1.0 / n) < 1e-15 and p[4] == np.log2(3)
+    k = _complete(40).features()
+    assert abs(k[5]) < 1e-12 and k[9] == 1.0 and k[8] == 1.0 and k[3] == 0.0
+    iso = G.from_edges(10, np.array([0], np.int32), np.array([1], np.int32)).features()
+    assert iso[7] == 0.8 and iso[10] == 0.2
+
+
+def test_feature_draws_follow_the_seeded_stream():
+    import ctypes
+    from indigo2suite_amd import _native
+    buf = (ctypes.c_uint64 * 8)()
+    assert _native.graph_lib().igx_feature_draws(5, 0xC1, 8, buf) == 0
+    first = list(buf)
+    assert _native.graph_lib().igx_feature_draws(5, 0xC1, 8, buf) == 0 and list(buf) == first
+    assert _native.graph_lib().igx_feature_draws(6, 0xC1, 8, buf) == 0 and list(buf) != first
+    assert len(set(first)) == 8

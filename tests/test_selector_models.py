@@ -18,7 +18,11 @@ MD = S.MODEL_DIR
 def test_models_and_datasets_match_registry():
     for algo in ALGOS:
         m = np.load(os.path.join(MD, f"selector_{algo}.npz"))
-        assert [str(x) for x in m["styles"]] == D.styles(algo)
+        # the models name their styles; a pick is looked up by name, so a
+        # model may know fewer styles than the library registers
+        names = [str(x) for x in m["styles"]]
+        reg = D.styles(algo)
+        assert set(names) <= set(reg) and names == sorted(names, key=reg.index)
         assert m["mean"].shape == (16,) and np.all(m["std"] > 0)
     for name in sorted(os.listdir(MD)):
         if not name.startswith(("train_", "test_")):
@@ -28,8 +32,8 @@ def test_models_and_datasets_match_registry():
         assert d["features"].shape == (n, 16) and n > 0
         for algo in ALGOS:
             t = d[f"times_{algo}"]
-            assert t.shape == (n, len(D.styles(algo)))
-            assert [str(x) for x in d[f"styles_{algo}"]] == D.styles(algo)
+            names = [str(x) for x in d[f"styles_{algo}"]]
+            assert t.shape == (n, len(names)) and set(names) <= set(D.styles(algo))
             # every graph has at least one measured style
             assert np.all(np.any(np.isfinite(t) & (t > 0), axis=1))
 
