@@ -42,7 +42,8 @@ typedef enum igx_status {
   IGX_ERR_NO_WEIGHTS = 7, /* algorithm needs eweight but the graph has none */
   IGX_ERR_STYLE = 8,      /* unknown style id for this algorithm */
   IGX_ERR_NODEV = 9,      /* no usable GPU */
-  IGX_ERR_LIMIT = 10      /* size beyond a documented limit */
+  IGX_ERR_LIMIT = 10,     /* size beyond a documented limit */
+  IGX_ERR_TIMEOUT = 11    /* run stopped at the caller's time limit (igx_set_time_limit) */
 } igx_status;
 
 /* ---- .egr file format (ECLgraph.h:52-96) ----------------------------------

@@ -118,6 +118,13 @@ int igx_profile_read(double* total_ms, int* launches);
  * max_iter = 100; those are the defaults). */
 int igx_set_pr_params(double eps, int max_iter);
 
+/* No reference counterpart.  Wall-clock budget of one run in seconds (0, the
+ * default, = none): a round-based style checks it between its batches of
+ * rounds and stops with IGX_ERR_TIMEOUT once past it, leaving partial output.
+ * For autotuners that time every style and need not wait for the slow ones;
+ * single-launch styles (triangle counting) do not check it. */
+int igx_set_time_limit(double seconds);
+
 /* ------------------------------------- reference-signature entry points ---- */
 /* Each takes the same arguments as the reference function named in its
  * comment, returns the elapsed seconds through `seconds`, and uses the style

@@ -46,8 +46,9 @@ IGX_OK = 0
 STATUS_NAMES = {
     0: "IGX_OK", 1: "IGX_ERR_ARG", 2: "IGX_ERR_IO", 3: "IGX_ERR_FORMAT", 4: "IGX_ERR_NOMEM",
     5: "IGX_ERR_HIP", 6: "IGX_ERR_UNSORTED", 7: "IGX_ERR_NO_WEIGHTS", 8: "IGX_ERR_STYLE",
-    9: "IGX_ERR_NODEV", 10: "IGX_ERR_LIMIT",
+    9: "IGX_ERR_NODEV", 10: "IGX_ERR_LIMIT", 11: "IGX_ERR_TIMEOUT",
 }
+IGX_ERR_TIMEOUT = 11
 
 
 class IgxError(RuntimeError):
@@ -137,6 +138,7 @@ def hip_lib():
         _sig(lib, "igx_run", ctypes.c_int, [ctypes.c_int, ctypes.c_int, G, ctypes.c_int, vp, ip,
                                             ctypes.POINTER(IgxRunInfo)])
         _sig(lib, "igx_set_pr_params", ctypes.c_int, [ctypes.c_double, ctypes.c_int])
+        _sig(lib, "igx_set_time_limit", ctypes.c_int, [ctypes.c_double])
         _sig(lib, "igx_profile_kernel", ctypes.c_int, [ctypes.c_char_p])
         _sig(lib, "igx_profile_read", ctypes.c_int, [dp, ip])
         _sig(lib, "igx_bfs_vertex", ctypes.c_int, [ctypes.c_int, G, ip, dp])

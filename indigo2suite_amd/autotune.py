@@ -19,6 +19,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import device
+from ._native import IGX_ERR_TIMEOUT, IgxError
 from .graph import Graph, chung_lu, erdos_renyi, grid2d, rmat
 
 FAMILIES = ("er", "rmat", "grid", "chunglu")
@@ -56,21 +57,44 @@ def build_from_spec(spec: dict) -> Graph:
 
 
 # Triangle-counting styles without orientation do O(d(v)^2) work per vertex
-# in one thread or wave: on a graph whose largest degree is in the tens of
-# thousands a single run takes minutes.  They are not run there; their time
-# is recorded as +inf ("too slow to measure"), which the selector learns as
-# the largest slowdown.
+# in one thread, wave or workgroup: on a graph whose largest degree is in the
+# tens of thousands a single run takes minutes, and in the measured sweeps
+# they never beat the oriented style once the sum of squared degrees passes
+# 1e9; the thread-per-vertex merge never won with a vertex above degree 256,
+# and there cost 100x the best.  They are not run there; their time is
+# recorded as +inf ("too slow to measure"), which the selector learns as the
+# largest slowdown and never picks.
 UNORIENTED_TC = ("v_merge_thread", "v_merge_wave", "e_bsearch_thread", "v_merge_block", "e_bsearch_wave",
                  "e_bsearch_block")
 TC_MAX_DEGREE = 8192
+TC_MAX_SUM_D2 = 1e9
+TC_THREAD_MERGE_MAX_DEGREE = 256
 
 
-def too_slow(algo: str, style: str, max_degree: int) -> bool:
-    return algo == "tc" and style in UNORIENTED_TC and max_degree > TC_MAX_DEGREE
+def too_slow(algo: str, style: str, max_degree: int, sum_d2: float = 0.0) -> bool:
+    if algo != "tc" or style not in UNORIENTED_TC:
+        return False
+    if style == "v_merge_thread" and max_degree > TC_THREAD_MERGE_MAX_DEGREE:
+        return True
+    return max_degree > TC_MAX_DEGREE or sum_d2 > TC_MAX_SUM_D2
 
 
-def sweep(dg: device.DeviceGraph, algo: str, src: int = 0, repeats: int = 3, cap_factor: float = 20.0,
-          styles: Optional[List[str]] = None, slow_seconds: float = 1.0, max_degree: int = 0) -> Dict[str, float]:
+def sum_sq_degrees(features: np.ndarray) -> float:
+    """n (variance + mean^2) from the features log2(n), mean and standard
+    deviation of the degrees (features 0, 2 and 3)."""
+    return float(2.0 ** features[0] * (features[3] ** 2 + features[2] ** 2))
+
+
+def sweep(dg: device.DeviceGraph, algo: str, src: int = 0, repeats: int = 3, cap_factor: float = 10.0,
+          styles: Optional[List[str]] = None, slow_seconds: float = 1.0, max_degree: int = 0,
+          repeat_factor: float = 3.0, limit_floor: float = 0.005, sum_d2: float = 0.0) -> Dict[str, float]:
+    """Seconds of every style (median of `repeats` runs).  The default style
+    runs first; every later run gets a time limit of cap_factor x the best
+    median so far plus `limit_floor` (igx_set_time_limit), and a style that
+    hits it is recorded as +inf (too slow: the selector learns the largest
+    slowdown).  Styles slower than repeat_factor x the best, or than
+    slow_seconds, are run once.  Output stays on the device (one buffer for
+    all runs)."""
     styles = list(styles or device.styles(algo))
     # the library's default style first: its time caps the repeats of the slow ones
     default = device.styles(algo)[device.default_entry_style(algo, 0)]
@@ -79,26 +103,45 @@ def sweep(dg: device.DeviceGraph, algo: str, src: int = 0, repeats: int = 3, cap
         styles.insert(0, default)
     times: Dict[str, float] = {}
     best = math.inf
-    for st in styles:
-        if too_slow(algo, st, max_degree):
-            times[st] = math.inf
-            continue
-        first = device.run(algo, dg, st, src).seconds
-        if first > cap_factor * best or first > slow_seconds:
-            times[st] = first
-            continue
-        ts = [first] + [device.run(algo, dg, st, src).seconds for _ in range(repeats - 1)]
-        times[st] = float(np.median(ts))
-        best = min(best, times[st])
+    out = device.DeviceBuffer(max(8 * dg.nodes, 64))
+
+    def timed(st):
+        limit = cap_factor * best + limit_floor if math.isfinite(best) else 0.0
+        device.set_time_limit(limit)
+        try:
+            return device.run_device(algo, dg, out, st, src).seconds
+        except IgxError as e:
+            if e.status == IGX_ERR_TIMEOUT:
+                return math.inf
+            raise
+        finally:
+            device.set_time_limit(0.0)
+
+    try:
+        for st in styles:
+            if too_slow(algo, st, max_degree, sum_d2):
+                times[st] = math.inf
+                continue
+            first = timed(st)
+            if not math.isfinite(first) or first > repeat_factor * best or first > slow_seconds:
+                times[st] = first
+                continue
+            ts = [first] + [timed(st) for _ in range(repeats - 1)]
+            times[st] = float(np.median(ts))
+            best = min(best, times[st])
+    finally:
+        out.free()
     return times
 
 
 def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_base: int = 100000,
             log_n_range=(12, 21), deg_range=(4, 32), repeats: int = 3, log=print, checkpoint=None,
-            prefetch: int = 4, checkpoint_every: int = 1) -> dict:
+            prefetch: int = 4, checkpoint_every: int = 1, max_seconds: float = 0.0) -> dict:
     """Features (computed on the device) and per-style times for `count`
     seeded graphs.  `log` gets a line per graph; checkpoint(partial_dataset)
-    is called every `checkpoint_every` graphs."""
+    is called every `checkpoint_every` graphs.  With max_seconds > 0 the
+    collection ends after the first graph that finishes past that many
+    seconds (the dataset then holds fewer than `count` graphs)."""
     device.init()
     feats = []
     times = {a: [] for a in algos}
@@ -122,13 +165,15 @@ def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_bas
                 pending[i + prefetch] = pool.submit(build_from_spec, specs_all[i + prefetch])
             src = g.max_degree_vertex()
             row = {}
-            dmax = int(np.diff(g.nindex).max()) if g.nodes > 0 else 0
+            deg = np.diff(g.nindex).astype(np.float64)
+            dmax = int(deg.max()) if g.nodes > 0 else 0
+            sum_d2 = float(np.sum(deg * deg))
             with device.DeviceGraph(g) as dg:
                 feat, _ = device.features_device(dg, spec["seed"])
                 for a in algos:
                     device.set_pr_params(1e-6, 100) if a == "pr" else device.set_pr_params(1e-4, 100)
                     ta = time.time()
-                    sw = sweep(dg, a, src, repeats, max_degree=dmax)
+                    sw = sweep(dg, a, src, repeats, max_degree=dmax, sum_d2=sum_d2)
                     row[a] = [sw[s] for s in device.styles(a)]
                     if log and g.nodes >= (1 << 20):
                         log(f"  graph {i + 1} {a}: {len(sw)} styles in {time.time() - ta:.1f}s")
@@ -139,8 +184,13 @@ def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_bas
             specs.append(spec)
             if log:
                 log(f"graph {i + 1}/{count} {spec['family']} n={g.nodes} m={g.edges} done at {time.time() - t0:.0f}s")
-            if checkpoint and ((i + 1) % checkpoint_every == 0 or i + 1 == count):
+            out_of_time = max_seconds > 0 and time.time() - t0 > max_seconds
+            if checkpoint and ((i + 1) % checkpoint_every == 0 or i + 1 == count or out_of_time):
                 checkpoint(dataset())
+            if out_of_time:
+                if log:
+                    log(f"time budget of {max_seconds:.0f}s spent after {i + 1} graphs")
+                break
     finally:
         device.set_pr_params(1e-4, 100)
         pool.shutdown(wait=True, cancel_futures=True)

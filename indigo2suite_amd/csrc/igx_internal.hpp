@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <chrono>
 #include <vector>
 
 #include "igx_graph.h"
@@ -40,6 +41,8 @@ struct Runtime {
   Ctl* d_ctl = nullptr;        // device control block
   Ctl* h_ctl = nullptr;        // pinned host mirror
   int persist_blocks = 0;      // grid of persistent kernels
+  double time_limit = 0.0;     // igx_set_time_limit: seconds per run, 0 = none
+  std::chrono::steady_clock::time_point t_start;  // host clock at timer_start
 };
 
 Runtime& rt();
@@ -59,7 +62,10 @@ int ws_get(int slot, size_t bytes, void** ptr);
 // (about 10 us, more than its device time), so the batch after the first
 // stays small: a run that needs a few rounds more than the first batch wastes
 // at most three submissions instead of a doubled batch.
+// After each batch the run also stops, with IGX_ERR_TIMEOUT, once the host
+// clock since timer_start() passes the time limit set by igx_set_time_limit.
 // Returns IGX_OK, IGX_ERR_LIMIT when max_rounds did not converge, or an error.
+bool past_time_limit();
 template <typename F>
 int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches, int first_batch = 2)
 {
@@ -76,10 +82,10 @@ int run_rounds(F launch, long long max_rounds, int* rounds_done, int* launches, 
     IGX_HIP(hipMemcpyAsync(R.h_ctl, R.d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, R.stream));
     IGX_HIP(hipStreamSynchronize(R.stream));
     if (R.h_ctl->done) break;
-    if (launched >= max_rounds) {
+    if (launched >= max_rounds || past_time_limit()) {
       if (rounds_done) *rounds_done = R.h_ctl->iter;
       if (launches) *launches = (int)launched;
-      return IGX_ERR_LIMIT;
+      return (launched >= max_rounds) ? IGX_ERR_LIMIT : IGX_ERR_TIMEOUT;
     }
     batch = (launched == k) ? 4 : ((batch < 64) ? batch * 2 : 64);
   }

@@ -376,7 +376,15 @@ void ktimer_end(const char* tag)
 int timer_start()
 {
   IGX_HIP(hipEventRecord(g_rt.ev_start, g_rt.stream));
+  g_rt.t_start = std::chrono::steady_clock::now();
   return IGX_OK;
+}
+
+bool past_time_limit()
+{
+  if (g_rt.time_limit <= 0.0) return false;
+  const std::chrono::duration<double> dt = std::chrono::steady_clock::now() - g_rt.t_start;
+  return dt.count() > g_rt.time_limit;
 }
 
 int timer_stop(double* seconds)
@@ -523,6 +531,7 @@ const char* igx_status_string(int status)
     case IGX_ERR_STYLE: return "unknown style";
     case IGX_ERR_NODEV: return "no GPU device";
     case IGX_ERR_LIMIT: return "size or round limit exceeded";
+    case IGX_ERR_TIMEOUT: return "run stopped at the time limit";
     default: return "unknown status";
   }
 }
@@ -704,6 +713,13 @@ int igx_profile_read(double* total_ms, int* launches)
   if (total_ms) *total_ms = sum;
   if (launches) *launches = (int)g_prof_used;
   g_prof_used = 0;
+  return IGX_OK;
+}
+
+int igx_set_time_limit(double seconds)
+{
+  if (!(seconds >= 0.0)) return IGX_ERR_ARG;
+  g_rt.time_limit = seconds;
   return IGX_OK;
 }
 

@@ -59,6 +59,12 @@ def default_entry_style(algo: str, edge_kind: int = 0) -> int:
     return lib().igx_get_entry_style(ALGOS[algo], 1 if edge_kind else 0)
 
 
+def set_time_limit(seconds: float) -> None:
+    """Wall-clock budget of each following run (0 = none); a round-based
+    style past it stops with IGX_ERR_TIMEOUT (igx_set_time_limit)."""
+    check(lib().igx_set_time_limit(seconds), "igx_set_time_limit")
+
+
 def set_pr_params(eps: float, max_iter: int) -> None:
     check(lib().igx_set_pr_params(eps, max_iter), "igx_set_pr_params")
 

@@ -6,7 +6,10 @@ hand-written HIP kernels of libindigo_hip.so (igx_mlp_*); the exhaustive
 autotuner (autotune.sweep) is both the fallback and the yardstick of regret.
 
 Targets: y_k = log(t_k / t_best), clipped at log(kClip); styles without a
-measurement are masked out of the loss.  Features are standardised with the
+measurement are masked out of the loss.  The library's default style is kept
+unless another style's predicted slowdown is lower by more than a margin
+chosen on held-back training graphs (fit_with_margin), so a wrong prediction
+costs less where the default is already the best style.  Features are standardised with the
 training set's mean and standard deviation, which are saved with the weights.
 """
 from __future__ import annotations
@@ -127,13 +130,17 @@ def targets_from_times(times: np.ndarray):
 
 
 class Selector:
-    def __init__(self, algo: str, styles: List[str], seed: int = 1):
+    def __init__(self, algo: str, styles: List[str], seed: int = 1, default: Optional[str] = None):
         self.algo = algo
         self.styles = list(styles)
         self.mean = None
         self.std = None
         self.net: Optional[Mlp] = None
         self.seed = seed
+        # the library's default style and the predicted log-slowdown margin
+        # another style must win by to be picked over it (fit_with_margin)
+        self.default = default
+        self.margin = 0.0
 
     def _norm(self, f: np.ndarray) -> np.ndarray:
         return ((np.asarray(f, np.float64) - self.mean) / self.std).astype(np.float32)
@@ -153,22 +160,68 @@ class Selector:
 
     def pick(self, features: np.ndarray) -> List[int]:
         """Style index with the lowest predicted slowdown, among the styles
-        the autotuner would run on that graph (feature 4 is log2(max degree + 1))."""
-        from .autotune import too_slow
+        the autotuner would run on that graph (feature 4 is log2(max degree +
+        1); the sum of squared degrees follows from features 0, 2 and 3)."""
+        from .autotune import sum_sq_degrees, too_slow
         f = np.atleast_2d(np.asarray(features, np.float64))
         scores = np.array(self.predict_scores(f), np.float64)
         for i in range(f.shape[0]):
             maxd = int(round(2.0 ** f[i, 4] - 1.0))
+            sd2 = sum_sq_degrees(f[i])
             for k, st in enumerate(self.styles):
-                if too_slow(self.algo, st, maxd):
+                if too_slow(self.algo, st, maxd, sd2):
                     scores[i, k] = np.inf
-        return list(np.argmin(scores, axis=1))
+        picks = list(np.argmin(scores, axis=1))
+        if self.default in self.styles and self.margin > 0.0:
+            d = self.styles.index(self.default)
+            for i, p in enumerate(picks):
+                if np.isfinite(scores[i, d]) and scores[i, d] - scores[i, p] <= self.margin:
+                    picks[i] = d
+        return picks
+
+    def fit_with_margin(self, features: np.ndarray, times: np.ndarray, default: str, epochs: int = 200,
+                        batch: int = 64, lr: float = 2e-3, val_frac: float = 0.2,
+                        margins=(0.0, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5), tol: float = 0.002) -> dict:
+        """Fit, then choose the margin by which a style's predicted slowdown
+        must beat the default's for it to be picked: the net is trained on a
+        seeded (1 - val_frac) share of the graphs, the largest margin whose
+        mean log slowdown against the best style on the rest is within `tol`
+        of the lowest is kept, and the net is trained again on every graph."""
+        f = np.asarray(features, np.float64)
+        t = np.asarray(times, np.float64)
+        self.default = default
+        n = f.shape[0]
+        order = np.random.default_rng(self.seed).permutation(n)
+        nval = max(1, int(round(val_frac * n))) if n > 1 else 0
+        val, tr = order[:nval], order[nval:]
+        scores = {}
+        if nval > 0 and len(tr) > 0:
+            self.fit(f[tr], t[tr], epochs=epochs, batch=batch, lr=lr)
+            tv = t[val]
+            ok = np.isfinite(tv) & (tv > 0)
+            best = np.where(ok, tv, np.inf).min(axis=1)
+            for mg in margins:
+                self.margin = mg
+                pk = self.pick(f[val])
+                chosen = tv[np.arange(len(val)), pk]
+                chosen = np.where(np.isfinite(chosen) & (chosen > 0), chosen, best * kClip)
+                scores[mg] = float(np.mean(np.log(chosen / best)))
+            self.net.free()
+            self.net = None
+            # the most conservative margin whose validation regret is within
+            # tol of the lowest: differences below that are timing noise
+            low = min(scores.values())
+            self.margin = max(mg for mg, v in scores.items() if v <= low + tol)
+        loss = self.fit(f, t, epochs=epochs, batch=batch, lr=lr)
+        return {"final_loss": loss, "margin": self.margin, "val_log_regret": scores}
 
     def evaluate(self, features: np.ndarray, times: np.ndarray, default_style: int) -> dict:
         t = np.asarray(times, np.float64)
         picks = self.pick(features)
         best = np.nanmin(np.where(np.isfinite(t), t, np.nan), axis=1)
         chosen = t[np.arange(t.shape[0]), picks]
+        # a pick that was too slow to measure counts as the clipped slowdown
+        chosen = np.where(np.isfinite(chosen) & (chosen > 0), chosen, best * kClip)
         regret = chosen / best - 1.0
         vs_default = t[:, default_style] / chosen
         return {"mean_regret": float(np.mean(regret)), "median_regret": float(np.median(regret)),
@@ -179,7 +232,8 @@ class Selector:
 
     def save(self, path: str) -> None:
         np.savez_compressed(path, algo=self.algo, styles=np.array(self.styles), mean=self.mean, std=self.std,
-                            params=self.net.params(), n_in=self.net.n_in)
+                            params=self.net.params(), n_in=self.net.n_in, default=self.default or "",
+                            margin=self.margin)
 
     @classmethod
     def load(cls, path: str) -> "Selector":
@@ -188,6 +242,9 @@ class Selector:
         s.mean, s.std = d["mean"], d["std"]
         s.net = Mlp(int(d["n_in"]), len(s.styles))
         s.net.set_params(d["params"])
+        if "default" in d.files and str(d["default"]):
+            s.default = str(d["default"])
+            s.margin = float(d["margin"])
         return s
 
 

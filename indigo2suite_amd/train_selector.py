@@ -34,9 +34,9 @@ def cmd_collect(a):
                             deg_range=tuple(a.deg), repeats=a.repeats,
                             log=lambda s: print(s, flush=True),
                             checkpoint=lambda d: autotune.save_dataset(a.out, d),
-                            checkpoint_every=a.checkpoint_every)
+                            checkpoint_every=a.checkpoint_every, max_seconds=a.max_seconds)
     autotune.save_dataset(a.out, data)
-    print(f"wrote {a.out}: {a.count} graphs in {time.time() - t0:.0f}s", flush=True)
+    print(f"wrote {a.out}: {len(data['features'])} graphs in {time.time() - t0:.0f}s", flush=True)
 
 
 def load_many(paths):
@@ -53,12 +53,15 @@ def cmd_fit(a):
     data = load_many(a.train)
     os.makedirs(a.out_dir, exist_ok=True)
     summary = {}
+    device.init()
     for algo in ALGOS:
         sel = Selector(algo, data["styles"][algo], seed=a.seed)
         t = time.time()
-        loss = sel.fit(data["features"], data["times"][algo], epochs=a.epochs, batch=a.batch, lr=a.lr)
+        default = device.styles(algo)[device.default_entry_style(algo, 0)]
+        res = sel.fit_with_margin(data["features"], data["times"][algo], default, epochs=a.epochs,
+                                  batch=a.batch, lr=a.lr)
         sel.save(os.path.join(a.out_dir, f"selector_{algo}.npz"))
-        summary[algo] = {"final_loss": loss, "train_seconds": round(time.time() - t, 2),
+        summary[algo] = {**res, "default": default, "train_seconds": round(time.time() - t, 2),
                          "graphs": int(data["features"].shape[0])}
         print(algo, summary[algo], flush=True)
         sel.net.free()
@@ -98,6 +101,7 @@ def main(argv=None):
     c.add_argument("--repeats", type=int, default=3)
     c.add_argument("--out", required=True)
     c.add_argument("--checkpoint-every", type=int, default=25, help="save the partial dataset every N graphs")
+    c.add_argument("--max-seconds", type=float, default=0.0, help="stop after the graph that passes this wall time")
     f = sub.add_parser("fit")
     f.add_argument("--train", required=True, nargs="+")
     f.add_argument("--out-dir", required=True)

@@ -172,9 +172,10 @@ def test_pr_full_size_all_styles(gpu, prtc):
 def test_tc_full_size(gpu, prtc):
     g, dg = prtc
     want = O.tc_forward(g)
-    maxdeg = int(g.degrees().max())
+    deg = g.degrees().astype(np.float64)
+    maxdeg = int(deg.max())
     for st in gpu.styles("tc"):
-        if autotune.too_slow("tc", st, maxdeg):
+        if autotune.too_slow("tc", st, maxdeg, float(np.sum(deg * deg))):
             continue                          # unoriented merges on a 64k-degree hub: quadratic
         _say("tc", g.name, st)
         assert gpu.run("tc", dg, st).out == want, (g.name, st)

@@ -145,3 +145,28 @@ def test_edge_styles_get_fresh_sp_after_free_and_reupload(gpu):
                 assert np.array_equal(D.run(algo, pb, st, 0).out, ref), (algo, st)
     finally:
         pb.free()
+
+
+def test_time_limit_stops_a_long_run_and_the_next_run_is_whole(gpu):
+    """igx_set_time_limit: a run needing ~100k rounds (deterministic push
+    along a path) stops at the limit with IGX_ERR_TIMEOUT; with the limit
+    cleared, the next runs on the same graph are complete and correct."""
+    from indigo2suite_amd._native import IGX_ERR_TIMEOUT, IgxError
+    n = 100000
+    g = G.from_edges(n, np.arange(n - 1, dtype=np.int32), np.arange(1, n, dtype=np.int32), name="path")
+    g = g.with_weights(1, 9, seed=4)
+    dg = D.DeviceGraph(g)
+    try:
+        _say("time limit: sssp v_topo_push_thread_determ on a 100k path, limit 0.05 s")
+        D.set_time_limit(0.05)
+        try:
+            with pytest.raises(IgxError) as ei:
+                D.run("sssp", dg, "v_topo_push_thread_determ", 0)
+            assert ei.value.status == IGX_ERR_TIMEOUT
+        finally:
+            D.set_time_limit(0.0)
+        _say("time limit cleared: sssp and bfs defaults")
+        assert np.array_equal(D.run("sssp", dg, None, 0).out, O.sssp_serial(g, 0))
+        assert np.array_equal(D.run("bfs", dg, None, 0).out, O.bfs_serial(g, 0))
+    finally:
+        dg.free()

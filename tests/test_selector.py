@@ -81,3 +81,32 @@ def test_training_reduces_loss_and_selector_beats_default(dev):
     finally:
         if sel.net:
             sel.net.free()
+
+
+def test_margin_keeps_the_default_unless_the_gain_is_clear(dev):
+    """Default style 0 is the best on most graphs by a hair (noise); styles 1-2
+    win by 3x where feature 0 is large.  After fit_with_margin the picks are
+    faster than the default overall and lose no more than the noise where the
+    default is the best."""
+    rng = np.random.default_rng(7)
+    n, K = 800, 3
+    f = rng.normal(size=(n, 16))
+    times = np.exp(rng.normal(0, 0.03, size=(n, K)))
+    times[:, 0] *= 0.99
+    big = f[:, 0] > 1.0
+    times[big, 0] *= 3.0
+    sel = S.Selector("bfs", ["s0", "s1", "s2"], seed=3)
+    try:
+        res = sel.fit_with_margin(f[:600], times[:600], "s0", epochs=120, batch=32, lr=2e-3)
+        print("margin fit:", res)
+        ev = sel.evaluate(f[600:], times[600:], default_style=0)
+        assert ev["geomean_speedup_vs_default"] > 1.05, (res, ev)
+        # where the default is the best in expectation, the picks cost at most
+        # the timing noise
+        noise = ~big[600:]
+        ev_noise = sel.evaluate(f[600:][noise], times[600:][noise], default_style=0)
+        assert ev_noise["geomean_speedup_vs_default"] > 0.97, (res, ev_noise)
+    finally:
+        if sel.net:
+            sel.net.free()
+
