@@ -30,8 +30,9 @@ the reference's global critical section) on a bounded sample: the same
 generator at scale 16; and the C++-threads restatement (codes/cpp family) on
 the same R-MAT 22 graph.
 
-selector: on the bench graph, for SSSP and CC, the learned selector's pick,
-the library default and the exhaustive sweep's best style, each timed.
+selector: on the bench graph, for BFS, SSSP and CC, the learned selector's
+pick (from features computed on the device, whose time is reported), the
+library default and the exhaustive sweep's best style, each timed.
 
 batched: the 512-graph batch (Erdos-Renyi and R-MAT halves, n log-uniform in
 [2^10, 2^18], degree U[4, 32]) as one captured persistent launch for BFS, CC
@@ -217,21 +218,26 @@ def cpu_cpp_threads(g, src, reps=3):
                        f"CC_CPP_V_Topo_Push restatement {t2:.3f} s ({itc} iterations), median of {reps}")}
 
 
-def selector_section(D, dg, g, src, algos=("sssp", "cc")):
+def selector_section(D, dg, g, src, algos=("bfs", "sssp", "cc")):
     """Learned pick vs the library default vs the exhaustive sweep's best, on
-    the bench graph, for the step's algorithms."""
+    the bench graph, for BFS and the step's algorithms.  The pick uses the
+    features computed on the device (the graph is already resident); their
+    cost is reported beside it."""
     from indigo2suite_amd import autotune, selector
     out = {}
-    dmax = int(g.degrees().max())
+    deg = g.degrees().astype(np.float64)
+    dmax, sum_d2 = int(deg.max()), float(np.sum(deg * deg))
+    feat_s = None
     for a in algos:
-        pick = selector.pick_style(g, a)
+        pick, feat_s = selector.pick_style_device(dg, a)
         default = D.styles(a)[D.default_entry_style(a, 0)]
-        sw = autotune.sweep(dg, a, src, repeats=3, max_degree=dmax)
+        sw = autotune.sweep(dg, a, src, repeats=3, max_degree=dmax, sum_d2=sum_d2)
         best = min(sw, key=sw.get)
         out[a] = {"pick": pick, "pick_s": round(sw[pick], 6), "default": default, "default_s": round(sw[default], 6),
                   "best": best, "best_s": round(sw[best], 6),
                   "regret_vs_best": round(sw[pick] / sw[best] - 1.0, 4),
                   "speedup_vs_default": round(sw[default] / sw[pick], 4), "styles_swept": len(sw)}
+    out["features_device_s"] = round(feat_s, 6) if feat_s is not None else None
     return out
 
 

@@ -29,7 +29,5 @@ int main(int argc, char* argv[])
   printf("number of connected components: %d\n", (int)s1.size());
   if (runveri) verify_components(g, label.data());
   else printf("verification turned off\n\n");
-  igx_graph_free_device(&d_g);
-  igx_host_graph_free(&g);
-  return 0;
+  return finish(&d_g, &g);
 }

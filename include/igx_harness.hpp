@@ -46,8 +46,19 @@ inline void check(int rc, const char* what)
 {
   if (rc != IGX_OK) {
     fprintf(stderr, "ERROR: %s failed: %s %s\n", what, igx_status_string(rc), igx_last_error());
+    igx_shutdown();
     exit(-1);
   }
+}
+
+// end of main(): release the library's device and pinned host resources
+// while the HIP runtime is fully alive, before static destruction starts
+inline int finish(igx_graph* d_g, igx_graph* g)
+{
+  if (d_g) igx_graph_free_device(d_g);
+  if (g) igx_host_graph_free(g);
+  check(igx_shutdown(), "igx_shutdown");
+  return 0;
 }
 
 // GPUinfo (bfs_vertex_cuda.h:63-73): select the device and print it

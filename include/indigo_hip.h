@@ -39,6 +39,13 @@ int igx_sync(void);
 /* Release cached device workspaces (kept between runs so that allocation
  * stays outside the timed region). */
 int igx_release_workspace(void);
+/* No reference counterpart.  Release everything the library holds on the
+ * device and the host (workspaces, the batched mode's buffers and captured
+ * graph, events, the control block and its pinned mirror, the stream), so a
+ * program can end with the HIP runtime holding nothing of the library's;
+ * the drop-in harnesses call it before main returns.  A later call
+ * initialises the library again. */
+int igx_shutdown(void);
 /* Text of the most recent HIP error recorded by the library. */
 const char* igx_last_error(void);
 const char* igx_status_string(int status);

@@ -29,7 +29,5 @@ int main(int argc, char* argv[])
     else printf("turn off verification\n\n");
   }
   report(kPlain, runtimes, runs, g.edges);
-  igx_graph_free_device(&d_g);
-  igx_host_graph_free(&g);
-  return 0;
+  return finish(&d_g, &g);
 }

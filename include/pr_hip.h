@@ -37,6 +37,5 @@ int main(int argc, char* argv[])
   for (int i = 0; i < g.nodes; i++) error += fabs(base_score + kDamp * incoming[i] - scores[i]);
   if (error < EPSILON) printf("All good.\n");
   else printf("Total Error: %f\n", error);
-  igx_host_graph_free(&g);
-  return 0;
+  return finish(nullptr, &g);
 }

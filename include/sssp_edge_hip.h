@@ -34,7 +34,5 @@ int main(int argc, char* argv[])
   printf("vertex %d has maximum distance %d\n", maxnode, distance[maxnode]);
   if (runveri) verify_distances(source, g, true, distance.data());
   else printf("turn off verification\n\n");
-  igx_graph_free_device(&d_g);
-  igx_host_graph_free(&g);
-  return 0;
+  return finish(&d_g, &g);
 }

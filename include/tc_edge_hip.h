@@ -34,7 +34,5 @@ int main(int argc, char* argv[])
     else printf("the pattern occurs %llu times\n\n", count);
   }
   igx_free(d_sp);
-  igx_graph_free_device(&d_g);
-  igx_host_graph_free(&g);
-  return 0;
+  return finish(&d_g, &g);
 }

@@ -119,6 +119,7 @@ def hip_lib():
         _sig(lib, "igx_device_info", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ip, ip])
         _sig(lib, "igx_sync", ctypes.c_int, [])
         _sig(lib, "igx_release_workspace", ctypes.c_int, [])
+        _sig(lib, "igx_shutdown", ctypes.c_int, [])
         _sig(lib, "igx_last_error", ctypes.c_char_p, [])
         _sig(lib, "igx_status_string", ctypes.c_char_p, [ctypes.c_int])
         _sig(lib, "igx_alloc", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t])

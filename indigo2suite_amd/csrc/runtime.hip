@@ -508,6 +508,28 @@ int igx_release_workspace(void)
   return batch_release();
 }
 
+int igx_shutdown(void)
+{
+  if (!g_rt.ready) return IGX_OK;
+  int rc = igx_release_workspace();
+  for (auto& e : g_prof_events) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  g_prof_events.clear();
+  g_prof_used = 0;
+  g_prof_open = false;
+  g_prof_tag[0] = 0;
+  if (hipEventDestroy(g_rt.ev_start) != hipSuccess) rc = IGX_ERR_HIP;
+  if (hipEventDestroy(g_rt.ev_stop) != hipSuccess) rc = IGX_ERR_HIP;
+  if (hipHostFree(g_rt.h_ctl) != hipSuccess) rc = IGX_ERR_HIP;
+  if (hipFree(g_rt.d_ctl) != hipSuccess) rc = IGX_ERR_HIP;
+  if (hipStreamDestroy(g_rt.stream) != hipSuccess) rc = IGX_ERR_HIP;
+  (void)hipGetLastError();
+  g_rt = Runtime{};
+  return rc;
+}
+
 const char* igx_last_error(void) { return g_last_error; }
 
 int igx_last_run_info(igx_run_info* info)

@@ -170,3 +170,34 @@ def test_time_limit_stops_a_long_run_and_the_next_run_is_whole(gpu):
         assert np.array_equal(D.run("bfs", dg, None, 0).out, O.bfs_serial(g, 0))
     finally:
         dg.free()
+
+
+_SHUTDOWN_SCRIPT = r"""
+import numpy as np, oracle as O
+from indigo2suite_amd import device as D, graph as G
+g = G.rmat(12, 8, seed=9).with_weights(1, 9, seed=9)
+for k in range(2):
+    D.init(0)
+    dg = D.DeviceGraph(g)
+    assert np.array_equal(D.run("sssp", dg, None, 0).out, O.sssp_serial(g, 0))
+    assert np.array_equal(D.run("bfs", dg, "e_data_push_thread", 0).out, O.bfs_serial(g, 0))
+    dg.free()
+    D.check(D.lib().igx_shutdown(), "igx_shutdown")
+    D.check(D.lib().igx_shutdown(), "igx_shutdown (again)")
+print("shutdown ok", flush=True)
+"""
+
+
+def test_shutdown_then_init_again_in_a_fresh_process(gpu):
+    """igx_shutdown releases the stream, events, control block and
+    workspaces; a later call initialises the library again.  Run in a child
+    process so this session's library state is untouched."""
+    import os
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    _say("shutdown / re-init in a child process")
+    out = subprocess.run([sys.executable, "-c", _SHUTDOWN_SCRIPT], cwd=repo, stdout=subprocess.PIPE,
+                         stderr=subprocess.STDOUT, text=True, timeout=300,
+                         env=dict(os.environ, PYTHONPATH=repo))
+    assert out.returncode == 0 and "shutdown ok" in out.stdout, out.stdout[-3000:]

@@ -21,6 +21,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CRASHED_FIXTURES = (18, 40)
 
 
+def _keep_log(name, text):
+    """Every driver run's whole output (each program's stdout and stderr, in
+    order) goes to gpurun_out/ at the repository root, which a GPU run hands
+    back, so a failing program can be read in full afterwards."""
+    d = os.path.join(os.path.dirname(HERE), "gpurun_out")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, name), "w") as f:
+        f.write(text)
+
+
 def _write_fixtures(d, exclude=()):
     with open(os.path.join(HERE, "golden", "small_inputs.json")) as f:
         fx = json.load(f)["fixtures"]
@@ -64,7 +74,7 @@ def test_driver_runs_programs_on_fixtures(tmp_path, gpu):
         code_dir = os.path.join(root, f"{algo}-hip")
         # per algorithm: the first program of each harness header and the
         # programs of the default entry styles keep the test short; edge-data
-        # programs are excluded (see test_driver_edge_data_programs below)
+        # programs are excluded (see test_driver_crashed_cases below)
         keep, headers = [], set()
         defaults = {gpu.styles(algo)[gpu.default_entry_style(algo, k)] for k in (0, 1)}
         for f in sorted(os.listdir(code_dir)):
@@ -83,8 +93,7 @@ def test_driver_runs_programs_on_fixtures(tmp_path, gpu):
                              cwd=str(tmp_path), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                              timeout=300, env=dict(os.environ, PYTHONPATH=run_codes.REPO_DIR))
         text = out.stdout
-        with open(os.path.join(str(tmp_path), f"driver_{algo}.log"), "w") as f:
-            f.write(text)
+        _keep_log(f"driver_{algo}.log", text)
         failed = [ln for ln in text.splitlines() if "exit status" in ln or "ERROR" in ln]
         assert not failed, (failed[:12], text[-2000:])
         assert "ERROR" not in text, text[-3000:]
@@ -115,6 +124,7 @@ def test_driver_crashed_cases(tmp_path, gpu):
         out = subprocess.run([sys.executable, "-m", "indigo2suite_amd.run_codes", code_dir, inputs, "3", "1"],
                              cwd=str(tmp_path), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                              timeout=600, env=dict(os.environ, PYTHONPATH=run_codes.REPO_DIR))
+        _keep_log(f"driver_crashed_{algo}.log", out.stdout)
         assert "exit status" not in out.stdout
 
 
