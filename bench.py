@@ -18,9 +18,10 @@ algorithmic bytes are the style-independent SSSP figure, 12 bytes per
 directed edge (neighbour id, weight, neighbour distance) and 12 per vertex
 (row offsets, distance read and write): 12m + 12n + 4 per run, spread over
 the run's launches.  achieved = those bytes / the summed launch time.  The
-bytes the kernel's executed work touches (4 per edge scanned, 8 more per
-edge relaxed, 12 per row visited, counted on the device) are reported beside
-it, as is the copy-calibrated HBM traffic per launch from rocprofv3.
+bytes the kernel's executed work touches (8 per edge scanned: id and weight;
+4 per neighbour distance gathered; 12 per row visited; counted on the device)
+are reported beside it, as is the copy-calibrated HBM traffic per launch
+from rocprofv3.
 
 cpu_baseline: the oracle's OpenMP restatement of the reference's fastest
 OpenMP styles (SSSP and CC vertex-based topology-driven pull) on the same
@@ -387,7 +388,7 @@ def main():
     alg_total = algorithmic_bytes("sssp", n, m) * args.steps
     achieved = alg_total / total_s * 1e-9 if total_s > 0 else 0.0
     # bytes of the work the launches executed, counted on the device
-    exec_total = 4 * work_e + 8 * work_r + 12 * work_v + 4 * launches
+    exec_total = 8 * work_e + 4 * work_r + 12 * work_v + 4 * launches
     traffic = measured_traffic(args.scale)
     alg_per_launch = alg_total / launches
     roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

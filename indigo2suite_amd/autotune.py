@@ -169,7 +169,7 @@ def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_bas
             dmax = int(deg.max()) if g.nodes > 0 else 0
             sum_d2 = float(np.sum(deg * deg))
             with device.DeviceGraph(g) as dg:
-                feat, _ = device.features_device(dg, spec["seed"])
+                feat, feat_s = device.features_device(dg, spec["seed"])
                 for a in algos:
                     device.set_pr_params(1e-6, 100) if a == "pr" else device.set_pr_params(1e-4, 100)
                     ta = time.time()
@@ -180,7 +180,7 @@ def collect(count: int, algos=("bfs", "sssp", "cc", "mis", "pr", "tc"), seed_bas
             feats.append(feat)
             for a in algos:
                 times[a].append(row[a])
-            spec.update({"n": g.nodes, "m": g.edges})
+            spec.update({"n": g.nodes, "m": g.edges, "features_s": feat_s})
             specs.append(spec)
             if log:
                 log(f"graph {i + 1}/{count} {spec['family']} n={g.nodes} m={g.edges} done at {time.time() - t0:.0f}s")
@@ -205,7 +205,49 @@ def save_dataset(path: str, data: dict) -> None:
     out["spec_n"] = np.array([s["n"] for s in data["specs"]])
     out["spec_m"] = np.array([s["m"] for s in data["specs"]])
     out["spec_family"] = np.array([s["family"] for s in data["specs"]])
+    out["spec_features_s"] = np.array([s.get("features_s", np.nan) for s in data["specs"]])
+    out["spec_seed"] = np.array([s["seed"] for s in data["specs"]])
     np.savez_compressed(path, **out)
+
+
+def retime(path: str, out: str, algo: str, styles: List[str], seed_base: int, log_n_range=(12, 21),
+           deg_range=(4, 32), repeats: int = 3, log=print, prefetch: int = 4) -> int:
+    """Time `styles` of `algo` again on the graphs of the dataset at `path`
+    (regenerated from the seeds that collect drew them from, checked against
+    the stored vertex and edge counts) and write the dataset with those
+    columns replaced to `out`.  For refreshing the labels of a style whose
+    kernels changed without re-timing every style.  Returns the graph count."""
+    d = dict(np.load(path))
+    names = [str(x) for x in d[f"styles_{algo}"]]
+    cols = [names.index(st) for st in styles]
+    count = d["features"].shape[0]
+    device.init()
+    specs = [graph_spec(i, seed_base, log_n_range, deg_range) for i in range(count)]
+    pool = ThreadPoolExecutor(max_workers=prefetch)
+    pending = {i: pool.submit(build_from_spec, specs[i]) for i in range(min(prefetch, count))}
+    t = d[f"times_{algo}"].copy()
+    t0 = time.time()
+    try:
+        for i in range(count):
+            g = pending.pop(i).result()
+            if i + prefetch < count:
+                pending[i + prefetch] = pool.submit(build_from_spec, specs[i + prefetch])
+            if g.nodes != int(d["spec_n"][i]) or g.edges != int(d["spec_m"][i]):
+                raise ValueError(f"graph {i} does not match the dataset (seed base or ranges differ)")
+            if algo == "pr":
+                device.set_pr_params(1e-6, 100)
+            with device.DeviceGraph(g) as dg:
+                sw = sweep(dg, algo, g.max_degree_vertex(), repeats, styles=list(styles))
+            for st, c in zip(styles, cols):
+                t[i, c] = sw[st]
+            if log and (i + 1) % 100 == 0:
+                log(f"retimed {i + 1}/{count} at {time.time() - t0:.0f}s")
+    finally:
+        device.set_pr_params(1e-4, 100)
+        pool.shutdown(wait=True, cancel_futures=True)
+    d[f"times_{algo}"] = t
+    np.savez_compressed(out, **d)
+    return count
 
 
 def load_dataset(path: str) -> dict:

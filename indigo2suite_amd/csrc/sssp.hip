@@ -434,12 +434,22 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
 // every value a round writes is at least the smallest distance of the
 // frontier it starts from (`settled`, the minimum the previous round lowered
 // anything to), so a row with dist <= settled cannot improve, now or later.
+// The same bound filters single edges before their neighbour's distance is
+// gathered: a candidate through an edge of weight w is at least settled + w
+// when the neighbour is in the frontier or was lowered in this round, and a
+// neighbour that is neither offers nothing new, so an edge with
+// settled + w >= dist[v] (dist[v] as read at the row's start; it only
+// decreases) cannot lower its row.  With uniform weights and distances that
+// soon fall well below the largest weight, this skips most gathers (a
+// synchronous model of R-MAT 20 from the hub: 3.6 m gathers in its pull
+// rounds without the filter, 0.8 m with it); ids and weights are streamed
+// together, so the filter adds no dependent load.
 // The next round pulls when the next frontier's edges exceed m / kAlphaS.  A
 // round that lowers nothing ends the run.
 //
 // Device work counters (algorithmic bytes of the round kernel): edges scanned
-// (4 B id each), edges relaxed (4 B weight + 4 B neighbour distance more),
-// rows visited (8 B offsets + 4 B distance).
+// (4 B id + 4 B weight each), edges relaxed, i.e. neighbour distances
+// gathered (4 B each), rows visited (8 B offsets + 4 B distance).
 constexpr int kAlphaS = 8;
 constexpr long long kQueueEdgesS = 1 << 16;
 // A pull round whose frontier holds more than this percentage of the edges
@@ -605,50 +615,51 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         const int ci = c0 + h * nwaves;
         en[h] = (ci < nbig) ? bigq[ci] : Entry{0, 0, 0, 0};
       }
+      int dv[kPair];
       #pragma unroll
       for (int h = 0; h < kPair; h++) {  // chunks of final rows are skipped
-        if (en[h].end > en[h].beg && dist[en[h].v] <= settled) en[h].end = en[h].beg;
+        dv[h] = (en[h].end > en[h].beg) ? dist[en[h].v] : 0;
+        if (dv[h] <= settled) en[h].end = en[h].beg;
       }
-      int u[kPair][kPer];
+      // ids and weights of the chunk's edges, loaded together
+      int u[kPair][kPer], wt[kPair][kPer];
       #pragma unroll
       for (int h = 0; h < kPair; h++) {
         #pragma unroll
         for (int k = 0; k < kPer; k++) {
           const int e = en[h].beg + k * kWave + lane;
           u[h][k] = (e < en[h].end) ? nlist[e] : -1;
+          wt[h][k] = (e < en[h].end) ? ew[e] : 0;
         }
       }
+      // weight filter (see kernel comment): only an edge with
+      // settled + w < dist[v] can lower its row
       bool hit[kPair][kPer];
-      if (all_front) {  // uniform
+      #pragma unroll
+      for (int h = 0; h < kPair; h++) {
         #pragma unroll
-        for (int h = 0; h < kPair; h++) {
-          #pragma unroll
-          for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0;
-        }
-      } else {
-        // the frontier words are loaded unconditionally (word 0 for lanes
-        // without an edge) so that the loads issue together
+        for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && settled + wt[h][k] < (long long)dv[h];
+      }
+      if (!all_front) {  // uniform
+        // the frontier words of the edges that pass are loaded together
         unsigned bw[kPair][kPer];
         #pragma unroll
         for (int h = 0; h < kPair; h++) {
           #pragma unroll
-          for (int k = 0; k < kPer; k++) bw[h][k] = cur[(u[h][k] >= 0 ? u[h][k] : 0) >> 5];
+          for (int k = 0; k < kPer; k++) bw[h][k] = hit[h][k] ? cur[u[h][k] >> 5] : 0u;
         }
         #pragma unroll
         for (int h = 0; h < kPair; h++) {
           #pragma unroll
-          for (int k = 0; k < kPer; k++) hit[h][k] = u[h][k] >= 0 && ((bw[h][k] >> (u[h][k] & 31)) & 1u);
+          for (int k = 0; k < kPer; k++) hit[h][k] = hit[h][k] && ((bw[h][k] >> (u[h][k] & 31)) & 1u);
         }
       }
-      // distances and weights of the hits, all issued before the first use
-      int du[kPair][kPer], wt[kPair][kPer];
+      // distances of the hits, all issued before the first use
+      int du[kPair][kPer];
       #pragma unroll
       for (int h = 0; h < kPair; h++) {
         #pragma unroll
-        for (int k = 0; k < kPer; k++) {
-          du[h][k] = hit[h][k] ? dist[u[h][k]] : kInf;
-          wt[h][k] = hit[h][k] ? ew[en[h].beg + k * kWave + lane] : 0;
-        }
+        for (int k = 0; k < kPer; k++) du[h][k] = hit[h][k] ? dist[u[h][k]] : kInf;
       }
       int cand[kPair];
       #pragma unroll
@@ -694,6 +705,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     // vary by more than 4x on R-MAT and drift with the vertex id; the long
     // rows' chunks above by little), so waves finish together.
     int* racc = s_acc[w];
+    int* rdv = s_excl[w];  // each lane's row distance (the push paths' scan space, unused here)
     int* rlist = s_rows[w];
     unsigned long long* heads = s_head[w];
     const int ntiles = (n + kWave - 1) / kWave;
@@ -719,6 +731,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         // rows at most `settled` are final (see above) and skip their edges
         const bool small = (v < n) && (re - rb <= kChunk) && dv > settled;
         racc[lane] = kInf;
+        rdv[lane] = dv;
         uint64_t todo = ballot(small);
         while (todo != 0) {  // uniform: one run of consecutive small rows
           const int first = __ffsll((unsigned long long)todo) - 1;
@@ -744,44 +757,45 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
               const int q = rb - c0;
               atomicOr(&heads[q >> 6], 1ull << (q & 63));
             }
-            int u[kPerLane];
+            // ids and weights of the slice, loaded together
+            int u[kPerLane], wt[kPerLane];
             #pragma unroll
             for (int k = 0; k < kPerLane; k++) {
               const int e = c0 + k * kWave + lane;
               u[k] = (e < c1) ? nlist[e] : -1;
-            }
-            bool hit[kPerLane];
-            if (all_front) {  // uniform
-              #pragma unroll
-              for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0;
-            } else {
-              unsigned bw[kPerLane];  // issued together, see the long rows above
-              #pragma unroll
-              for (int k = 0; k < kPerLane; k++) bw[k] = cur[(u[k] >= 0 ? u[k] : 0) >> 5];
-              #pragma unroll
-              for (int k = 0; k < kPerLane; k++) hit[k] = u[k] >= 0 && ((bw[k] >> (u[k] & 31)) & 1u);
-            }
-            int du[kPerLane], wt[kPerLane];
-            #pragma unroll
-            for (int k = 0; k < kPerLane; k++) {
-              du[k] = hit[k] ? dist[u[k]] : kInf;
-              wt[k] = hit[k] ? ew[c0 + k * kWave + lane] : 0;
+              wt[k] = (e < c1) ? ew[e] : 0;
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            // each position's row (lane of the tile), then the weight filter
+            // against that row's distance
             const uint64_t upto = ~0ull >> (kWave - 1 - lane);  // positions 0..lane of a word
+            int row[kPerLane];
+            bool hit[kPerLane];
             int pk = before;
             #pragma unroll
             for (int k = 0; k < kPerLane; k++) {
-              relaxed += hit[k] ? 1 : 0;
               const uint64_t hw = heads[k];
-              if (du[k] != kInf) {
-                const int row = rlist[pk + __popcll(hw & upto) - 1];
-                atomicMin(&racc[row], du[k] + wt[k]);
-              }
+              row[k] = (u[k] >= 0) ? rlist[pk + __popcll(hw & upto) - 1] : 0;
+              hit[k] = u[k] >= 0 && settled + wt[k] < (long long)rdv[row[k]];
               pk += __popcll(hw);
             }
             before = pk;
+            if (!all_front) {  // uniform
+              unsigned bw[kPerLane];  // issued together, see the long rows above
+              #pragma unroll
+              for (int k = 0; k < kPerLane; k++) bw[k] = hit[k] ? cur[u[k] >> 5] : 0u;
+              #pragma unroll
+              for (int k = 0; k < kPerLane; k++) hit[k] = hit[k] && ((bw[k] >> (u[k] & 31)) & 1u);
+            }
+            int du[kPerLane];
+            #pragma unroll
+            for (int k = 0; k < kPerLane; k++) du[k] = hit[k] ? dist[u[k]] : kInf;
+            #pragma unroll
+            for (int k = 0; k < kPerLane; k++) {
+              relaxed += hit[k] ? 1 : 0;
+              if (du[k] != kInf) atomicMin(&racc[row[k]], du[k] + wt[k]);
+            }
           }
         }
         __builtin_amdgcn_wave_barrier();

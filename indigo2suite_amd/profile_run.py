@@ -54,11 +54,12 @@ def main(argv=None) -> int:
     if a.algo == "pr":
         D.set_pr_params(1e-6, 100)
     style = D.styles(a.algo)[D.style_id(a.algo, a.style)]
-    secs, iters = [], []
+    secs, iters, work = [], [], []
     for _ in range(a.runs):
         info = D.run_device(a.algo, dg, out, style, src)
         secs.append(info.seconds)
         iters.append(info.iterations)
+        work.append((int(info.edges_processed), int(info.edges_relaxed)))
     D.lib().igx_sync()
     if a.copy_probe > 0:
         nb = a.copy_probe << 20
@@ -68,6 +69,8 @@ def main(argv=None) -> int:
         y.free()
     print(json.dumps({"algo": a.algo, "graph": a.graph, "style": style, "n": g.nodes, "m": g.edges,
                       "runs": a.runs, "seconds": secs, "median_s": float(np.median(secs)), "iterations": iters,
+                      "edges_scanned_per_m": [round(w[0] / max(g.edges, 1), 3) for w in work],
+                      "edges_relaxed_per_m": [round(w[1] / max(g.edges, 1), 3) for w in work],
                       "copy_probe_mib": a.copy_probe}), flush=True)
     out.free()
     dg.free()

@@ -3,12 +3,15 @@ evaluate them on held-out graphs.
 
   python -m indigo2suite_amd.train_selector collect --count N --seed-base S \
         [--log-n LO HI] [--deg LO HI] --out FILE.npz
+  python -m indigo2suite_amd.train_selector retime --data FILE.npz --out FILE2.npz --algo A --styles S [S ...] \
+        --seed-base S [--log-n LO HI] [--deg LO HI]
   python -m indigo2suite_amd.train_selector fit --train FILE.npz [FILE.npz ...] --out-dir DIR
   python -m indigo2suite_amd.train_selector eval --test FILE.npz [FILE.npz ...] --model-dir DIR [--json OUT]
 
 `collect` runs the exhaustive autotuner (every registered style of every
 algorithm, median of three runs) on seeded graphs from four families;
-`fit` trains one MLP per algorithm with the HIP kernels; `eval` reports, per
+`retime` times some styles again on a dataset's graphs (after their kernels
+changed) and replaces their columns; `fit` trains one MLP per algorithm with the HIP kernels; `eval` reports, per
 algorithm, the regret of the learned pick against the exhaustive best and its
 speedup over the fixed default style (the library's entry style).
 """
@@ -37,6 +40,13 @@ def cmd_collect(a):
                             checkpoint_every=a.checkpoint_every, max_seconds=a.max_seconds)
     autotune.save_dataset(a.out, data)
     print(f"wrote {a.out}: {len(data['features'])} graphs in {time.time() - t0:.0f}s", flush=True)
+
+
+def cmd_retime(a):
+    t0 = time.time()
+    n = autotune.retime(a.data, a.out, a.algo, a.styles, a.seed_base, tuple(a.log_n), tuple(a.deg),
+                        repeats=a.repeats, log=lambda s: print(s, flush=True))
+    print(f"wrote {a.out}: {a.algo} {a.styles} retimed on {n} graphs in {time.time() - t0:.0f}s", flush=True)
 
 
 def load_many(paths):
@@ -102,6 +112,15 @@ def main(argv=None):
     c.add_argument("--out", required=True)
     c.add_argument("--checkpoint-every", type=int, default=25, help="save the partial dataset every N graphs")
     c.add_argument("--max-seconds", type=float, default=0.0, help="stop after the graph that passes this wall time")
+    r = sub.add_parser("retime", help="time some styles again on a dataset's graphs, replacing their columns")
+    r.add_argument("--data", required=True)
+    r.add_argument("--out", required=True)
+    r.add_argument("--algo", required=True, choices=ALGOS)
+    r.add_argument("--styles", required=True, nargs="+")
+    r.add_argument("--seed-base", type=int, required=True)
+    r.add_argument("--log-n", type=float, nargs=2, default=(12, 21))
+    r.add_argument("--deg", type=float, nargs=2, default=(4, 32))
+    r.add_argument("--repeats", type=int, default=3)
     f = sub.add_parser("fit")
     f.add_argument("--train", required=True, nargs="+")
     f.add_argument("--out-dir", required=True)
@@ -114,7 +133,7 @@ def main(argv=None):
     e.add_argument("--model-dir", required=True)
     e.add_argument("--json")
     a = ap.parse_args(argv)
-    {"collect": cmd_collect, "fit": cmd_fit, "eval": cmd_eval}[a.cmd](a)
+    {"collect": cmd_collect, "retime": cmd_retime, "fit": cmd_fit, "eval": cmd_eval}[a.cmd](a)
 
 
 if __name__ == "__main__":
