@@ -450,7 +450,10 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
 // Device work counters (algorithmic bytes of the round kernel): edges scanned
 // (4 B id + 4 B weight each), edges relaxed, i.e. neighbour distances
 // gathered (4 B each), rows visited (8 B offsets + 4 B distance).
-constexpr int kAlphaS = 8;
+#ifndef IGX_SSSP_ALPHA
+#define IGX_SSSP_ALPHA 8
+#endif
+constexpr int kAlphaS = IGX_SSSP_ALPHA;
 constexpr long long kQueueEdgesS = 1 << 16;
 // A pull round whose frontier holds more than this percentage of the edges
 // reads every neighbour's distance instead of testing its frontier bit first:
