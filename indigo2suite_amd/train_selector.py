@@ -89,8 +89,11 @@ def cmd_eval(a):
         ev = sel.evaluate(data["features"], data["times"][algo], default_style=default)
         t = np.asarray(data["times"][algo])
         best = np.argmin(t, axis=1)
+        tb = t[np.arange(len(best)), best]
         ev["default_style"] = sel.styles[default]
-        ev["default_regret_mean"] = float(np.mean(t[:, default] / t[np.arange(len(best)), best] - 1.0))
+        ev["default_regret_mean"] = float(np.mean(t[:, default] / tb - 1.0))
+        ev["best_speedup_vs_default"] = float(np.exp(np.mean(np.log(t[:, default] / tb))))
+        ev["margin"] = sel.margin
         ev["best_styles"] = [sel.styles[b] for b in best]
         out[algo] = ev
         print(algo, {k: v for k, v in ev.items() if not isinstance(v, list)}, flush=True)
@@ -98,6 +101,24 @@ def cmd_eval(a):
     if a.json:
         with open(a.json, "w") as f:
             json.dump(out, f, indent=1)
+    if a.markdown:
+        with open(a.markdown, "w") as f:
+            f.write(markdown_table(out))
+
+
+def markdown_table(ev: dict) -> str:
+    """One row per algorithm: the learned pick's regret against the tuned best,
+    the default's, and speedups over the default (geometric means)."""
+    lines = ["| algorithm | default style | margin | mean regret, learned | max | within 10% of best | "
+             "mean regret, default | speedup vs default, learned | speedup vs default, tuned best | "
+             "faster than default on |",
+             "|---|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for algo, e in ev.items():
+        lines.append(f"| {algo} | `{e['default_style']}` | {e['margin']:.2f} | {100 * e['mean_regret']:.1f}% | "
+                     f"{100 * e['max_regret']:.0f}% | {100 * e['frac_within_10pct']:.0f}% | "
+                     f"{100 * e['default_regret_mean']:.1f}% | {e['geomean_speedup_vs_default']:.3f} | "
+                     f"{e['best_speedup_vs_default']:.3f} | {100 * e['frac_faster_than_default']:.0f}% |")
+    return "\n".join(lines) + "\n"
 
 
 def main(argv=None):
@@ -132,6 +153,7 @@ def main(argv=None):
     e.add_argument("--test", required=True, nargs="+")
     e.add_argument("--model-dir", required=True)
     e.add_argument("--json")
+    e.add_argument("--markdown", help="write the per-algorithm table as markdown")
     a = ap.parse_args(argv)
     {"collect": cmd_collect, "retime": cmd_retime, "fit": cmd_fit, "eval": cmd_eval}[a.cmd](a)
 

@@ -1,8 +1,8 @@
 """Shipped selector models and timing datasets: their style lists match the
-library's registry (CPU), and on the held-out graphs of 2^19 to 2^20.5
-vertices the trained picks stay within a few percent of the exhaustive best
-and no slower than the default style on average (GPU: the MLP runs on the
-device)."""
+library's registry (CPU), and on the two held-out sets (graphs of 2^14 to
+2^22 vertices; graphs of 2^23 vertices with average degree 16) the trained
+picks stay within a few percent of the exhaustive best and, on average, no
+slower than the default style (GPU: the MLP runs on the device)."""
 import os
 
 import numpy as np
@@ -38,16 +38,23 @@ def test_models_and_datasets_match_registry():
             assert np.all(np.any(np.isfinite(t) & (t > 0), axis=1))
 
 
+# held-out sets: (file, minimum geometric-mean speedup over the default style)
+HELD_OUT = (("test_2100000.npz", 1.0),   # 152 graphs, 2^14 to 2^22 vertices, degree U[4, 32]
+            ("test_2000000.npz", 0.98))  # 50 graphs at 2^23 vertices, degree 16: the defaults are the tuned best
+
+
 @pytest.mark.gpu
-def test_shipped_models_on_held_out_graphs(gpu):
-    d = np.load(os.path.join(MD, "test_900000.npz"))
+@pytest.mark.parametrize("name,min_speedup", HELD_OUT)
+def test_shipped_models_on_held_out_graphs(gpu, name, min_speedup):
+    d = np.load(os.path.join(MD, name))
     for algo in ALGOS:
         sel = S.Selector.load(os.path.join(MD, f"selector_{algo}.npz"))
         try:
             dflt = D.default_entry_style(algo)
             ev = sel.evaluate(d["features"], d[f"times_{algo}"], dflt)
-            assert ev["geomean_speedup_vs_default"] >= 0.98, (algo, ev)
-            assert ev["frac_within_10pct"] >= 0.9, (algo, ev)
+            print(name, algo, {k: v for k, v in ev.items() if k != "picks"}, flush=True)
+            assert ev["geomean_speedup_vs_default"] >= min_speedup, (algo, ev)
+            assert ev["frac_within_10pct"] >= 0.85, (algo, ev)
             assert ev["mean_regret"] <= 0.05, (algo, ev)
         finally:
             sel.net.free()
