@@ -180,6 +180,8 @@ struct Ctl {
   double res_last;       // PR residual of the last completed round
   int n;                 // vertex count (for heuristics)
   int in_bits;           // BFS: the current frontier is held by its bitmap only
+  int push_cut;          // SSSP: a push round relaxes only frontier vertices below this distance (0: all)
+  int seeded;            // SSSP: the run has had its partial push round
 };
 
 // Round epilogue.  Ending a round inside the round kernel ("the last block to

@@ -454,6 +454,10 @@ __global__ void __launch_bounds__(kBlock) sssp_bal_round(const int* __restrict__
 #define IGX_SSSP_ALPHA 8
 #endif
 constexpr int kAlphaS = IGX_SSSP_ALPHA;
+#ifndef IGX_SSSP_SEED_DIV
+#define IGX_SSSP_SEED_DIV 32
+#endif
+constexpr int kSeedDiv = IGX_SSSP_SEED_DIV;  // 0: no partial push round
 constexpr long long kQueueEdgesS = 1 << 16;
 // A pull round whose frontier holds more than this percentage of the edges
 // reads every neighbour's distance instead of testing its frontier bit first:
@@ -509,11 +513,13 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   __shared__ unsigned long long s_head[kWavesPerBlock][kSlice / kWave];
   __shared__ Entry s_ent[kWavesPerBlock][kWave];
   __shared__ long long s_red[kWavesPerBlock * kStats];
+  __shared__ double s_redd[2][kWavesPerBlock];
   if (round_done(ctl)) return;
   const int iter = ctl->iter;
   const int mode = ctl->mode;
   const int in_b = ctl->in_bits;
-  const bool out_q = (mode == 0) && (ctl->edges_next <= kQueueEdgesS);
+  // a partial push round (push_cut != 0) always writes its frontier as a bitmap
+  const bool out_q = (mode == 0) && (ctl->edges_next <= kQueueEdgesS) && ctl->push_cut == 0;
   const long long settled = ctl->aux_ll[1];
   const bool all_front = (mode == 1) && ctl->edges_next * 100 > m * kAllFrontPct;
   const int p = iter & 1;
@@ -534,6 +540,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   }
   long long front_e = 0, scanned = 0, relaxed = 0, rows = 0;
   long long low = LLONG_MAX;  // smallest distance this round lowered something to
+  double lsum = 0.0, lcnt = 0.0;  // sum and count of the values this round lowered distances to
   // first marker of u in the next frontier: counts its edges, returns its row
   auto mark = [&](int u, int& ub, int& ue) -> bool {
     const unsigned bit = 1u << (u & 31);
@@ -541,6 +548,18 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     ub = nindex[u];
     ue = nindex[u + 1];
     front_e += ue - ub;
+    return true;
+  };
+  const int cut = (mode == 0) ? ctl->push_cut : 0;
+  // a frontier vertex at or above the cut of a partial push round is not
+  // pushed: it stays in the frontier (next bitmap, its edges counted)
+  auto keep_pending = [&](int v) -> bool {
+    if (cut == 0) return false;
+    const int dv = dist[v];
+    if (dv < cut) return false;
+    int ub = 0, ue = 0;
+    low = min(low, (long long)dv);
+    mark(v, ub, ue);
     return true;
   };
   if (mode == 0) {
@@ -551,8 +570,25 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       const int u = nlist[e];
       bool first = false;
       int ub = 0, ue = 0;
+      if (cut != 0) {
+        // partial push round (its output is a bitmap, never a queue): the
+        // atomics' old values are not needed, so they do not return one and
+        // the wave does not wait on them -- most targets start at infinity
+        // and the hubs among them take many candidates at once.  A lowered
+        // vertex adds 1 instead of its degree to the next frontier's edges
+        // (the pending part of the frontier is counted exactly).
+        if (dist[u] > nd) {
+          atomic_min_relaxed(&dist[u], nd);
+          atomicOr(&next[u >> 5], 1u << (u & 31));
+          low = min(low, (long long)nd);
+          front_e += 1;
+        }
+        return;
+      }
       if (dist[u] > nd && atomic_min_relaxed(&dist[u], nd) > nd) {
         low = min(low, (long long)nd);
+        lsum += nd;
+        lcnt += 1.0;
         first = mark(u, ub, ue);
       }
       if (out_q) push_vertex_chunks(q_mine, out_count, first && ue > ub, u, ub, ue);
@@ -561,7 +597,12 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       const SegPrefix pre = segq_prefix(ctl, p);
       if (blockIdx.x == 0 && threadIdx.x == 0) rows = pre.pre[kGroups];
       wave_expand_with(pre.pre[kGroups], gw, nwaves, s_excl[w], s_ent[w],
-                       [&](int i) { return segq_at(q_in, pre, i); }, relax);
+                       [&](int i) {
+                         Entry en = segq_at(q_in, pre, i);
+                         if (en.end > en.beg && keep_pending(en.v)) en.end = en.beg;
+                         return en;
+                       },
+                       relax);
     } else {
       for_each_bitmap_tile(cur, n, gw, nwaves, [&](int v0, bool in) {
         const int v = v0 + lane;
@@ -569,7 +610,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         if (in) {
           const int b = nindex[v], e = nindex[v + 1];
           rows += 1;
-          if (e - b <= kChunk) ent = Entry{v, b, e, 0};
+          if (!keep_pending(v) && e - b <= kChunk) ent = Entry{v, b, e, 0};
         }
         const int len = ent.end - ent.beg;
         const int incl = wave_incl_scan(len);
@@ -598,7 +639,8 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
                        [&](int i) {
                          Entry en = bigq[i];
-                         if (!bit_of(cur, en.v)) en.end = en.beg;
+                         // rows of pending vertices were kept by the tile walk above
+                         if (!bit_of(cur, en.v) || (cut != 0 && dist[en.v] >= cut)) en.end = en.beg;
                          return en;
                        },
                        relax);
@@ -687,6 +729,8 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
           if (en[h].end > en[h].beg && cand[h] < dist[v] && atomic_min_relaxed(&dist[v], cand[h]) > cand[h]) {
             int ub, ue;
             low = min(low, (long long)cand[h]);
+            lsum += cand[h];
+            lcnt += 1.0;
             mark(v, ub, ue);
           }
         }
@@ -810,6 +854,8 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
           dist[v] = acc;
           front_e += len;
           low = min(low, (long long)acc);
+          lsum += acc;
+          lcnt += 1.0;
         }
         // this wave owns the tile's bits except those of rows longer than
         // kChunk, which other waves' chunk walks may set concurrently
@@ -823,6 +869,8 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     }
   }
   round_partials(ctl, front_e, scanned, relaxed, rows, s_red, low);
+  round_partial_d(ctl, 0, lsum, s_redd[0]);
+  round_partial_d(ctl, 1, lcnt, s_redd[1]);
 }
 
 // Round decision (run by round_finalize after each sssp_do_round).
@@ -831,7 +879,7 @@ struct FinSsspDo {
   __device__ void operator()(Ctl* ctl, const RoundTotals& t) const
   {
     const long long ef = t.s[kStatFrontierEdges];
-    const bool out_q = (ctl->mode == 0) && (ctl->edges_next <= kQueueEdgesS);
+    const bool out_q = (ctl->mode == 0) && (ctl->edges_next <= kQueueEdgesS) && ctl->push_cut == 0;
     const int p = ctl->iter & 1;
     ctl->work_edges += t.s[kStatScanned];
     ctl->aux_ll[0] += t.s[kStatRelaxed];
@@ -847,8 +895,30 @@ struct FinSsspDo {
     ctl->edges_next = ef;
     // ef == 0: nothing was lowered, or only vertices without edges, from
     // which nothing can propagate
-    if (ef == 0) ctl->done = 1;
-    else ctl->mode = (ef > m / kAlphaS) ? 1 : 0;
+    ctl->push_cut = 0;
+    if (ef == 0) {
+      ctl->done = 1;
+    } else if (ef > m / kAlphaS) {
+      // The first frontier large enough for a pull, reached by pushes: most
+      // rows are still unreached, so a pull would gather a distance for
+      // nearly every edge (no edge can be filtered against an infinite
+      // row).  Instead, one push round from the closest part of the
+      // frontier (distance below lowest + (mean - lowest) / kSeedDiv) gives
+      // most rows a finite distance first; the rest of the frontier stays
+      // in it for the pull that follows.
+      const double cnt = t.d[1];
+      const double lowest = (double)t.s[kStatMin];
+      const double mean = (cnt > 0.0) ? t.d[0] / cnt : 0.0;
+      if (kSeedDiv > 0 && ctl->mode == 0 && !ctl->seeded && mean - lowest >= 2.0 * kSeedDiv) {
+        ctl->push_cut = (int)(lowest + (mean - lowest) / kSeedDiv) + 1;
+        ctl->seeded = 1;
+        ctl->mode = 0;
+      } else {
+        ctl->mode = 1;
+      }
+    } else {
+      ctl->mode = 0;
+    }
   }
 };
 
@@ -1054,7 +1124,7 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
                            g->eweight, g->nodes, (long long)g->edges, dist, q0, q1, (const Entry*)pb,
                            (const int*)pc, (unsigned*)pm, words, grab, R.d_ctl);
         ktimer_end("sssp_do_round");
-        launch_finalize(R.persist_blocks, 0xF | (1 << kStatMin), FinSsspDo{(long long)g->edges});
+        launch_finalize(R.persist_blocks, 0xF | (1 << kStatMin) | (3 << kStats), FinSsspDo{(long long)g->edges});
       },
       sssp_round_cap(g), &rounds, &launches, 8);
   IGX_TRY(timer_stop(&info->seconds));
