@@ -215,10 +215,20 @@ def retime(path: str, out: str, algo: str, styles: List[str], seed_base: int, lo
     """Time `styles` of `algo` again on the graphs of the dataset at `path`
     (regenerated from the seeds that collect drew them from, checked against
     the stored vertex and edge counts) and write the dataset with those
-    columns replaced to `out`.  For refreshing the labels of a style whose
-    kernels changed without re-timing every style.  Returns the graph count."""
+    columns replaced, or added for a style registered since, to `out`.  For
+    refreshing the labels of a style whose kernels changed without re-timing
+    every style.  Returns the graph count."""
     d = dict(np.load(path))
     names = [str(x) for x in d[f"styles_{algo}"]]
+    reg = device.styles(algo)
+    for st in styles:  # a style registered after the dataset was made gets a new column
+        if st not in names:
+            names.append(st)
+            d[f"times_{algo}"] = np.concatenate([d[f"times_{algo}"], np.full((d["features"].shape[0], 1), np.nan)],
+                                                axis=1)
+    if names != [x for x in reg if x in names]:
+        raise ValueError("dataset styles are not in registry order")
+    d[f"styles_{algo}"] = np.array(names)
     cols = [names.index(st) for st in styles]
     count = d["features"].shape[0]
     device.init()

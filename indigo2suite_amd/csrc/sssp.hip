@@ -14,6 +14,10 @@
 //                   chunks expanded by wave-level merge path
 //   v_topo_{push,pull}_{thread,wave}_determ  V_Topo_{Push,Pull}_Determ
 //                   siblings: Jacobi rounds over two distance buffers
+//   v_do_balanced   (no reference counterpart) direction-optimizing push/pull
+//                   rounds on bitmap frontiers with weight-filtered pulls and a
+//                   partial push round before the first pull over a hub
+//                   frontier
 #include "igx_internal.hpp"
 #include "igx_frontier.hpp"
 
@@ -458,6 +462,10 @@ constexpr int kAlphaS = IGX_SSSP_ALPHA;
 #define IGX_SSSP_SEED_DIV 32
 #endif
 constexpr int kSeedDiv = IGX_SSSP_SEED_DIV;  // 0: no partial push round
+#ifndef IGX_SSSP_HUB_RATIO
+#define IGX_SSSP_HUB_RATIO 4.0
+#endif
+constexpr double kHubRatio = IGX_SSSP_HUB_RATIO;
 constexpr long long kQueueEdgesS = 1 << 16;
 // A pull round whose frontier holds more than this percentage of the edges
 // reads every neighbour's distance instead of testing its frontier bit first:
@@ -869,8 +877,10 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     }
   }
   round_partials(ctl, front_e, scanned, relaxed, rows, s_red, low);
-  round_partial_d(ctl, 0, lsum, s_redd[0]);
-  round_partial_d(ctl, 1, lcnt, s_redd[1]);
+  if (kSeedDiv > 0 && mode == 0 && !ctl->seeded) {  // uniform: only push rounds before the partial push need the mean
+    round_partial_d(ctl, 0, lsum, s_redd[0]);
+    round_partial_d(ctl, 1, lcnt, s_redd[1]);
+  }
 }
 
 // Round decision (run by round_finalize after each sssp_do_round).
@@ -909,7 +919,13 @@ struct FinSsspDo {
       const double cnt = t.d[1];
       const double lowest = (double)t.s[kStatMin];
       const double mean = (cnt > 0.0) ? t.d[0] / cnt : 0.0;
-      if (kSeedDiv > 0 && ctl->mode == 0 && !ctl->seeded && mean - lowest >= 2.0 * kSeedDiv) {
+      // Only where the frontier is made of hubs (its edges per lowered
+      // vertex exceed kHubRatio times the graph's average degree: a power-law
+      // frontier next to the source), since there the pull would scan most
+      // edges for few reached rows; on a frontier of ordinary vertices the
+      // extra round costs more than it saves.
+      const bool hubs = ef > kHubRatio * cnt * ((double)m / ctl->n);
+      if (kSeedDiv > 0 && ctl->mode == 0 && !ctl->seeded && hubs && mean - lowest >= 2.0 * kSeedDiv) {
         ctl->push_cut = (int)(lowest + (mean - lowest) / kSeedDiv) + 1;
         ctl->seeded = 1;
         ctl->mode = 0;
