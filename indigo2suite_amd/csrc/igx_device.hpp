@@ -182,6 +182,7 @@ struct Ctl {
   int in_bits;           // BFS: the current frontier is held by its bitmap only
   int push_cut;          // SSSP: a push round relaxes only frontier vertices below this distance (0: all)
   int seeded;            // SSSP: the run has had its partial push round
+  int force_all;         // SSSP: the next pull reads every neighbour (set after the partial push round)
 };
 
 // Round epilogue.  Ending a round inside the round kernel ("the last block to

@@ -529,7 +529,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   // a partial push round (push_cut != 0) always writes its frontier as a bitmap
   const bool out_q = (mode == 0) && (ctl->edges_next <= kQueueEdgesS) && ctl->push_cut == 0;
   const long long settled = ctl->aux_ll[1];
-  const bool all_front = (mode == 1) && ctl->edges_next * 100 > m * kAllFrontPct;
+  const bool all_front = (mode == 1) && (ctl->edges_next * 100 > m * kAllFrontPct || ctl->force_all);
   const int p = iter & 1;
   const SegQ q_in = p ? q1 : q0;
   const SegQ q_out = p ? q0 : q1;
@@ -559,15 +559,17 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
     return true;
   };
   const int cut = (mode == 0) ? ctl->push_cut : 0;
-  // a frontier vertex at or above the cut of a partial push round is not
-  // pushed: it stays in the frontier (next bitmap, its edges counted)
+  // A frontier vertex at or above the cut of a partial push round is not
+  // pushed: it stays in the frontier.  The pull after a partial push round
+  // reads every neighbour (force_all), so neither the pending vertices nor
+  // the ones the round lowers are marked in a bitmap; only the smallest
+  // distance of the next frontier (pending ones included) is kept.
   auto keep_pending = [&](int v) -> bool {
     if (cut == 0) return false;
     const int dv = dist[v];
     if (dv < cut) return false;
-    int ub = 0, ue = 0;
     low = min(low, (long long)dv);
-    mark(v, ub, ue);
+    front_e += 1;
     return true;
   };
   if (mode == 0) {
@@ -579,15 +581,14 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
       bool first = false;
       int ub = 0, ue = 0;
       if (cut != 0) {
-        // partial push round (its output is a bitmap, never a queue): the
-        // atomics' old values are not needed, so they do not return one and
-        // the wave does not wait on them -- most targets start at infinity
-        // and the hubs among them take many candidates at once.  A lowered
-        // vertex adds 1 instead of its degree to the next frontier's edges
-        // (the pending part of the frontier is counted exactly).
+        // partial push round: the atomic's old value is not needed, so it
+        // does not return one and the wave does not wait on it -- most
+        // targets start at infinity and the hubs among them take many
+        // candidates at once.  No bitmap mark (see keep_pending); a lowered
+        // vertex counts 1 in the round's frontier statistic, which only has
+        // to be nonzero.
         if (dist[u] > nd) {
           atomic_min_relaxed(&dist[u], nd);
-          atomicOr(&next[u >> 5], 1u << (u & 31));
           low = min(low, (long long)nd);
           front_e += 1;
         }
@@ -905,8 +906,15 @@ struct FinSsspDo {
     ctl->edges_next = ef;
     // ef == 0: nothing was lowered, or only vertices without edges, from
     // which nothing can propagate
+    const bool was_partial = ctl->push_cut != 0;
     ctl->push_cut = 0;
-    if (ef == 0) {
+    ctl->force_all = 0;
+    if (was_partial) {
+      // the pull that follows a partial push reads every neighbour: the
+      // round's frontier (pending and lowered vertices) is not in a bitmap
+      ctl->mode = 1;
+      ctl->force_all = 1;
+    } else if (ef == 0) {
       ctl->done = 1;
     } else if (ef > m / kAlphaS) {
       // The first frontier large enough for a pull, reached by pushes: most
