@@ -183,6 +183,7 @@ struct Ctl {
   int push_cut;          // SSSP: a push round relaxes only frontier vertices below this distance (0: all)
   int seeded;            // SSSP: the run has had its partial push round
   int force_all;         // SSSP: the next pull reads every neighbour (set after the partial push round)
+  int active;            // workgroups that take part in the next round (0: all); round_finalize reduces that many
 };
 
 // Round epilogue.  Ending a round inside the round kernel ("the last block to
@@ -342,6 +343,8 @@ __global__ void __launch_bounds__(kFinBlock) round_finalize(Ctl* ctl, int nparts
   for (int k = 0; k < kStatsD; k++) accd[k] = 0.0;
   const long long* pl = ctl_parts(ctl);
   const double* pd = ctl_parts_d(ctl);
+  // a round run by fewer workgroups (Ctl::active) left that many partials
+  if (ctl->active > 0 && ctl->active < nparts) nparts = ctl->active;
   for (int b0 = threadIdx.x; b0 < nparts; b0 += 2 * kFinBlock) {
     const int b1 = b0 + kFinBlock;  // two partials per statistic, loaded together
     #pragma unroll

@@ -467,6 +467,7 @@ constexpr int kSeedDiv = IGX_SSSP_SEED_DIV;  // 0: no partial push round
 #endif
 constexpr double kHubRatio = IGX_SSSP_HUB_RATIO;
 constexpr long long kQueueEdgesS = 1 << 16;
+constexpr long long kEdgesPerBlock = 2048;  // frontier edges per workgroup of a push round from a queue
 // A pull round whose frontier holds more than this percentage of the edges
 // reads every neighbour's distance instead of testing its frontier bit first:
 // a neighbour outside the frontier only offers a candidate its row has
@@ -541,9 +542,13 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const int gw = blockIdx.x * kWavesPerBlock + w;
-  const int nwaves = gridDim.x * kWavesPerBlock;
+  // a push round from a small queue runs on the first `active` workgroups
+  // only (set by the round before); the others leave at once
+  const int active = (ctl->active > 0) ? ctl->active : (int)gridDim.x;
+  if ((int)blockIdx.x >= active) return;
+  const int nwaves = active * kWavesPerBlock;
   {
-    const int stride = gridDim.x * kBlock;
+    const int stride = active * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < words; i += stride) stale[i] = 0u;
   }
   long long front_e = 0, scanned = 0, relaxed = 0, rows = 0;
@@ -887,6 +892,7 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
 // Round decision (run by round_finalize after each sssp_do_round).
 struct FinSsspDo {
   long long m;
+  int nblocks;  // the round kernel's grid
   __device__ void operator()(Ctl* ctl, const RoundTotals& t) const
   {
     const long long ef = t.s[kStatFrontierEdges];
@@ -943,6 +949,15 @@ struct FinSsspDo {
     } else {
       ctl->mode = 0;
     }
+    // a push from a queue (the frontier was small) needs few workgroups:
+    // one per kEdgesPerBlock edges of the frontier, at least 16
+    int act = 0;
+    if (!ctl->done && ctl->mode == 0 && ctl->in_bits == 0) {
+      const long long want = (ef + kEdgesPerBlock - 1) / kEdgesPerBlock;
+      act = (int)min(max(want, 16LL), (long long)nblocks);
+      if (act >= nblocks) act = 0;
+    }
+    ctl->active = act;
   }
 };
 
@@ -1148,7 +1163,8 @@ int style_do(const igx_graph* g, int src, void* out, const void*, igx_run_info* 
                            g->eweight, g->nodes, (long long)g->edges, dist, q0, q1, (const Entry*)pb,
                            (const int*)pc, (unsigned*)pm, words, grab, R.d_ctl);
         ktimer_end("sssp_do_round");
-        launch_finalize(R.persist_blocks, 0xF | (1 << kStatMin) | (3 << kStats), FinSsspDo{(long long)g->edges});
+        launch_finalize(R.persist_blocks, 0xF | (1 << kStatMin) | (3 << kStats),
+                        FinSsspDo{(long long)g->edges, R.persist_blocks});
       },
       sssp_round_cap(g), &rounds, &launches, 8);
   IGX_TRY(timer_stop(&info->seconds));
