@@ -335,9 +335,17 @@ extern "C" int igx_graph_features_device(const igx_graph* g, uint64_t seed, doub
   Runtime& R = rt();
   const int n = g->nodes;
   const long long m = g->edges;
-  hipEvent_t t0, t1;
-  IGX_HIP(hipEventCreate(&t0));
-  IGX_HIP(hipEventCreate(&t1));
+  struct Events {  // destroyed on every return path
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events()
+    {
+      if (a) hipEventDestroy(a);
+      if (b) hipEventDestroy(b);
+    }
+  } ev;
+  IGX_HIP(hipEventCreate(&ev.a));
+  IGX_HIP(hipEventCreate(&ev.b));
+  hipEvent_t t0 = ev.a, t1 = ev.b;
   IGX_HIP(hipEventRecord(t0, R.stream));
 
   // degree pass: the histogram size needs the maximum degree first
@@ -419,8 +427,6 @@ extern "C" int igx_graph_features_device(const igx_graph* g, uint64_t seed, doub
   IGX_HIP(hipEventSynchronize(t1));
   float ms = 0.0f;
   IGX_HIP(hipEventElapsedTime(&ms, t0, t1));
-  IGX_HIP(hipEventDestroy(t0));
-  IGX_HIP(hipEventDestroy(t1));
   if (seconds) *seconds = ms * 1e-3;
 
   // the same formulas as igx_graph_features
