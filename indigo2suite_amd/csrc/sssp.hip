@@ -529,7 +529,8 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
   const int in_b = ctl->in_bits;
   // a partial push round (push_cut != 0) always writes its frontier as a bitmap
   const bool out_q = (mode == 0) && (ctl->edges_next <= kQueueEdgesS) && ctl->push_cut == 0;
-  const long long settled = ctl->aux_ll[1];
+  // (a round that lowered nothing leaves LLONG_MAX; no distance exceeds INT_MAX)
+  const long long settled = min(ctl->aux_ll[1], (long long)INT_MAX);
   const bool all_front = (mode == 1) && (ctl->edges_next * 100 > m * kAllFrontPct || ctl->force_all);
   const int p = iter & 1;
   const SegQ q_in = p ? q1 : q0;
