@@ -230,6 +230,48 @@ __device__ __forceinline__ void wave_expand_with(int qsize, int gw, int nwaves, 
   }
 }
 
+// The same walk with the entries dealt out round-robin: lane l of wave gw
+// takes entry gw + (64 s + l) * nwaves in step s, so consecutive entries go
+// to consecutive waves.  For a list in which the live entries come in long
+// runs (the chunks of a few hub rows among the chunks of every long row),
+// the consecutive grouping above hands a whole run to one wave; here a run
+// of c chunks spreads over c waves.
+template <typename Fetch, typename F>
+__device__ __forceinline__ void wave_expand_strided(int qsize, int gw, int nwaves, int* lds_excl, Entry* lds_ent,
+                                                    Fetch fetch, F visit)
+{
+  const int lane = lane_id();
+  const long long step = (long long)kWave * nwaves;
+  for (long long base = gw; base < qsize; base += step) {
+    const long long idx = base + (long long)lane * nwaves;
+    Entry ent = Entry{0, 0, 0, 0};
+    if (idx < qsize) ent = fetch((int)idx);
+    const int len = ent.end - ent.beg;
+    const int incl = wave_incl_scan(len);
+    const int total = wave_bcast(incl, kWave - 1);
+    if (total == 0) continue;  // uniform
+    __builtin_amdgcn_wave_barrier();
+    lds_excl[lane] = incl - len;
+    lds_ent[lane] = ent;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (int tb = 0; tb < total; tb += kWave) {
+      const int t = tb + lane;
+      if (t < total) {
+        int lo = 0, hi = kWave;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (lds_excl[mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        const Entry& en = lds_ent[lo];
+        visit(en, en.beg + (t - lds_excl[lo]));
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 template <typename F>
 __device__ __forceinline__ void wave_expand(const Entry* __restrict__ q, int qsize, int gw, int nwaves,
                                             int* lds_excl, Entry* lds_ent, F visit)

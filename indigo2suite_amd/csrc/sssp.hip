@@ -592,8 +592,11 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         // targets start at infinity and the hubs among them take many
         // candidates at once.  No bitmap mark (see keep_pending); a lowered
         // vertex counts 1 in the round's frontier statistic, which only has
-        // to be nonzero.
-        if (dist[u] > nd) {
+        // to be nonzero.  The test before the atomic reads the distance from
+        // L2 (an agent-scope load): a copy in this CU's L1 would not see
+        // other CUs' lowered values, and a hub target would take an atomic
+        // for every candidate.
+        if (load_relaxed(&dist[u]) > nd) {
           atomic_min_relaxed(&dist[u], nd);
           low = min(low, (long long)nd);
           front_e += 1;
@@ -651,14 +654,16 @@ __global__ void __launch_bounds__(kBlock) sssp_do_round(const int* __restrict__ 
         }
         __builtin_amdgcn_wave_barrier();
       });
-      wave_expand_with(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
-                       [&](int i) {
-                         Entry en = bigq[i];
-                         // rows of pending vertices were kept by the tile walk above
-                         if (!bit_of(cur, en.v) || (cut != 0 && dist[en.v] >= cut)) en.end = en.beg;
-                         return en;
-                       },
-                       relax);
+      // chunks dealt out round-robin: the live ones are the runs of chunks
+      // of the frontier's long rows, which would otherwise fall to a few waves
+      wave_expand_strided(*bigq_count, gw, nwaves, s_excl[w], s_ent[w],
+                          [&](int i) {
+                            Entry en = bigq[i];
+                            // rows of pending vertices were kept by the tile walk above
+                            if (!bit_of(cur, en.v) || (cut != 0 && dist[en.v] >= cut)) en.end = en.beg;
+                            return en;
+                          },
+                          relax);
     }
   } else {
     // rows longer than kChunk (most of the edges of a power-law graph): one

@@ -244,8 +244,9 @@ __global__ void __launch_bounds__(kBlock) tc_orient_scatter(const int* __restric
 // wave per v builds an open-addressing hash set of N+(v) in LDS (the table is
 // the power of two >= 2 |N+(v)|, multiplicative hashing, linear probing), then
 // walks the concatenated out-lists of v's out-neighbours (wave merge path, so
-// short and long lists share the lanes evenly), kIlp elements per lane per
-// step with their loads issued together, and probes each element once.  An
+// short and long lists share the lanes evenly), kIlp consecutive elements
+// per lane per step with their loads issued together (one owner search per
+// lane and step), and probes each element once.  An
 // out-list longer than kHashMax / 2 is hashed in passes over consecutive
 // slices of it (the list is sorted, so a slice is a value range and elements
 // outside it skip the probe).
@@ -336,18 +337,27 @@ __global__ void __launch_bounds__(kBlock) tc_oriented_count(const int* __restric
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           for (int tb = 0; tb < total; tb += kIlp * kWave) {
+            // the lane's kIlp consecutive positions t0 .. t0 + kIlp - 1: a
+            // binary search finds the owner of t0 (the last k with excl[k] <=
+            // t0), and each later position moves it forward past the lists
+            // that end before it (excl is non-decreasing, and excl[k + 1] >
+            // t for the owner k, so t - excl[k] is inside k's list)
+            const int t0 = tb + lane * kIlp;
             int x[kIlp];
+            int lo = 0;
+            if (t0 < total) {
+              int hi = kWave;
+              while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (excl[mid] <= t0) lo = mid;
+                else hi = mid;
+              }
+            }
             #pragma unroll
             for (int q = 0; q < kIlp; q++) {
-              const int t = tb + q * kWave + lane;
-              int lo = 0;
-              if (t < total) {
-                int hi = kWave;  // last k with excl[k] <= t
-                while (hi - lo > 1) {
-                  const int mid = (lo + hi) >> 1;
-                  if (excl[mid] <= t) lo = mid;
-                  else hi = mid;
-                }
+              const int t = t0 + q;
+              if (q > 0 && t < total) {
+                while (lo + 1 < kWave && excl[lo + 1] <= t) lo++;
               }
               x[q] = (t < total) ? ebeg[lo] + (t - excl[lo]) : -1;
             }
