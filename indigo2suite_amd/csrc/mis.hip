@@ -12,6 +12,9 @@
 //   v_data_pull_*  V_Data_Pull siblings: worklist of undecided vertices
 //   e_topo_thread  [C-MET]:57-127 / [O-MISe]:52-94 edge pass + vertex pass
 //   e_data_thread  [C-MED]:45-213 worklist of edges, edge pass + vertex pass
+//   e_data_thread_{copy,merge}determ  MIS_OMP_E_Data_..._{Copy,Merge}Determ:
+//                  the same passes over a second status buffer, published
+//                  after each pass
 //   v_resume_pull  (no reference counterpart) pull over the outranking
 //                  neighbours only, resuming where the last round stopped
 #include "igx_internal.hpp"
@@ -306,6 +309,104 @@ __global__ void __launch_bounds__(kBlock) mis_edata_last(int* __restrict__ statu
   if (v < n && status[v] == kUndecided) status[v] = kIncluded;
 }
 
+// Determ forms of the edge worklist style: MIS_OMP_E_Data_IntType_Atomic_
+// CopyDeterm_Default_NoFieldBug.cpp OMPmis_edge() and its MergeDeterm sibling
+// (include/mis_edge_omp.h:151-163 updateFromWorklist).  Each pass reads
+// `status` only and writes its decisions into `status_n`; a publish kernel
+// after the pass copies them to `status`: every vertex (Copy), or the
+// endpoints of the round's worklist edges, the only vertices a pass can
+// change (Merge).  No read in a pass sees a write of the same pass, so every
+// round's decisions are fixed by the round before.  status_n equals status
+// at the start of every pass.
+__global__ void __launch_bounds__(kBlock) mis_determ_edge(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                           const int* __restrict__ status,
+                                                           int* __restrict__ status_n, int* __restrict__ lost,
+                                                           const int* __restrict__ wl0, const int* __restrict__ wl1,
+                                                           Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int p = ctl->iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  const int stamp = ctl->iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+    const int e = wl_in[idx];
+    const int src = sp[e];
+    const int dst = nlist[e];
+    const int ss = status[src];
+    const int sd = status[dst];
+    if (ss == kIncluded) {
+      status_n[dst] = kExcluded;
+    } else if (sd == kIncluded) {
+      status_n[src] = kExcluded;
+    } else if (ss == kUndecided && sd == kUndecided) {
+      const bool dst_wins = outranks(mis_priority(dst), dst, mis_priority(src), src);
+      lost[dst_wins ? src : dst] = stamp;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) mis_determ_vertex(const int* __restrict__ nlist, const int* __restrict__ sp,
+                                                             const int* __restrict__ status,
+                                                             int* __restrict__ status_n, const int* __restrict__ lost,
+                                                             int* __restrict__ wl0, int* __restrict__ wl1, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int p = ctl->iter & 1;
+  const int count = ctl->wl_count[p];
+  const int* wl_in = p ? wl1 : wl0;
+  int* wl_out = p ? wl0 : wl1;
+  int* out_count = &ctl->wl_count[p ^ 1];
+  const int stamp = ctl->iter + 1;
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock; base < count; base += stride) {  // uniform per wave
+    const int idx = base + threadIdx.x;
+    bool keep = false;
+    int e = 0;
+    if (idx < count) {
+      e = wl_in[idx];
+      const int src = sp[e];
+      const int dst = nlist[e];
+      const int ss = status[src];
+      const int sd = status[dst];
+      if (ss == kUndecided && lost[src] != stamp) status_n[src] = kIncluded;
+      if (sd == kUndecided && lost[dst] != stamp) status_n[dst] = kIncluded;
+      keep = ss == kUndecided || sd == kUndecided;
+    }
+    const int slot = wave_push_slot(out_count, keep);
+    if (keep) wl_out[slot] = e;
+  }
+}
+
+// status <- status_n after a pass: all n vertices (Copy) or the endpoints of
+// the round's input worklist edges (Merge)
+template <bool MERGE>
+__global__ void __launch_bounds__(kBlock) mis_determ_publish(const int* __restrict__ nlist,
+                                                              const int* __restrict__ sp,
+                                                              const int* __restrict__ status_n,
+                                                              int* __restrict__ status, int n,
+                                                              const int* __restrict__ wl0,
+                                                              const int* __restrict__ wl1, Ctl* ctl)
+{
+  if (round_done(ctl)) return;
+  const int stride = gridDim.x * kBlock;
+  if constexpr (MERGE) {
+    const int p = ctl->iter & 1;
+    const int count = ctl->wl_count[p];
+    const int* wl_in = p ? wl1 : wl0;
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < count; idx += stride) {
+      const int e = wl_in[idx];
+      const int src = sp[e];
+      const int dst = nlist[e];
+      status[src] = status_n[src];
+      status[dst] = status_n[dst];
+    }
+  } else {
+    for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) status[v] = status_n[v];
+  }
+}
+
 // ------------------------------------------------------- resumable pull ----
 // Only the neighbours that outrank v matter for v joining the set: v joins
 // once all of them are excluded.  A vertex that joins excludes all its
@@ -569,6 +670,50 @@ int style_edata(const igx_graph* g, int, void* out, const void* aux, igx_run_inf
   return rc;
 }
 
+template <bool MERGE>
+int style_edata_determ(const igx_graph* g, int, void* out, const void* aux, igx_run_info* info)
+{
+  Runtime& R = rt();
+  int* status = (int*)out;
+  const int* sp = (const int*)aux;
+  const size_t wl_items = (size_t)(g->edges > 0 ? g->edges : 1);
+  void *pl = nullptr, *p0 = nullptr, *p1 = nullptr, *pn = nullptr;
+  IGX_TRY(ws_get(WS_A, sizeof(int) * (size_t)g->nodes, &pl));
+  IGX_TRY(ws_get(WS_B, sizeof(int) * wl_items, &p0));
+  IGX_TRY(ws_get(WS_C, sizeof(int) * wl_items, &p1));
+  IGX_TRY(ws_get(WS_D, sizeof(int) * (size_t)g->nodes, &pn));
+  int* lost = (int*)pl;
+  int* status_n = (int*)pn;
+  IGX_TRY(timer_start());
+  IGX_TRY(ctl_reset(g->nodes));
+  const dim3 vb(blocks_for(g->nodes, kBlock));
+  hipLaunchKernelGGL(mis_init, vb, dim3(kBlock), 0, R.stream, status, g->nodes);
+  hipLaunchKernelGGL(mis_init, vb, dim3(kBlock), 0, R.stream, status_n, g->nodes);
+  hipLaunchKernelGGL(mis_init, vb, dim3(kBlock), 0, R.stream, lost, g->nodes);
+  hipLaunchKernelGGL(mis_edata_seed, dim3(R.persist_blocks), dim3(kBlock), 0, R.stream, g->nlist, sp, g->edges,
+                     (int*)p0, R.d_ctl);
+  int rounds = 0, launches = 0;
+  const int rc = run_rounds(
+      [&](long long) {
+        const dim3 grid(R.persist_blocks), blk(kBlock);
+        hipLaunchKernelGGL(mis_determ_edge, grid, blk, 0, R.stream, g->nlist, sp, (const int*)status, status_n, lost,
+                           (const int*)p0, (const int*)p1, R.d_ctl);
+        hipLaunchKernelGGL((mis_determ_publish<MERGE>), grid, blk, 0, R.stream, g->nlist, sp,
+                           (const int*)status_n, status, g->nodes, (const int*)p0, (const int*)p1, R.d_ctl);
+        hipLaunchKernelGGL(mis_determ_vertex, grid, blk, 0, R.stream, g->nlist, sp, (const int*)status, status_n,
+                           (const int*)lost, (int*)p0, (int*)p1, R.d_ctl);
+        hipLaunchKernelGGL((mis_determ_publish<MERGE>), grid, blk, 0, R.stream, g->nlist, sp,
+                           (const int*)status_n, status, g->nodes, (const int*)p0, (const int*)p1, R.d_ctl);
+        launch_fin_worklist();
+      },
+      (long long)g->nodes + 2, &rounds, &launches);
+  hipLaunchKernelGGL(mis_edata_last, vb, dim3(kBlock), 0, R.stream, status, g->nodes);
+  IGX_TRY(timer_stop(&info->seconds));
+  info->iterations = rounds;
+  info->launches = launches;
+  return rc;
+}
+
 const StyleEntry kStyles[] = {
     {"v_topo_push_thread", style_vtopo_push<false>, 0},
     {"v_topo_push_wave", style_vtopo_push<true>, 0},
@@ -577,6 +722,8 @@ const StyleEntry kStyles[] = {
     {"e_topo_thread", style_etopo, 1},
     {"v_resume_pull", style_resume_pull, 0},
     {"e_data_thread", style_edata, 1},
+    {"e_data_thread_copydeterm", style_edata_determ<false>, 1},
+    {"e_data_thread_mergedeterm", style_edata_determ<true>, 1},
 };
 
 }  // namespace
