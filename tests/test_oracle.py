@@ -114,6 +114,11 @@ def test_oracle_matches_reference_binaries_fresh(seed):
     assert np.array_equal(O.sssp_serial(g, src), O.ref_run("sssp_edge_data", g, src))
     assert np.array_equal(O.cc_omp_push(g)[0], O.ref_run("cc_push", g))
     assert np.array_equal(O.mis_greedy(g), O.ref_run("mis_vertex", g))
+    # the reference's edge worklist Determ forms (statuses double-buffered per
+    # pass) reach the same greedy set as the vertex form
+    for v in ("mis_edata_copy", "mis_edata_merge"):
+        if O.ref_available(v):
+            assert np.array_equal(O.mis_greedy(g), O.ref_run(v, g)), v
     assert O.tc_serial(g) == O.ref_run("tc_edge", g)
 
 
