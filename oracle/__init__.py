@@ -284,6 +284,10 @@ REF_OUT = {  # name -> (dtype, is_scalar)
     "cc_push": (np.int32, False), "cc_pull": (np.int32, False),
     "mis_vertex": (np.uint8, False), "mis_edge": (np.uint8, False),
     "mis_edata_copy": (np.uint8, False), "mis_edata_merge": (np.uint8, False),
+    "bfs_vtopo_pull_determ": (np.int32, False), "bfs_vdata_push_determ": (np.int32, False),
+    "sssp_edata_determ": (np.int32, False), "sssp_vtopo_pull_determ": (np.int32, False),
+    "sssp_vdata_push_determ": (np.int32, False),
+    "cc_vtopo_pull_determ": (np.int32, False), "cc_vdata_push_determ": (np.int32, False),
     "pr_pull": (np.float32, False), "tc_vertex": (np.int64, True), "tc_edge": (np.int64, True),
     "cpp_bfs": (np.int32, False), "cpp_sssp": (np.int32, False), "cpp_cc": (np.int32, False),
     "cpp_mis": (np.uint8, False), "cpp_pr": (np.float32, False), "cpp_tc": (np.int64, True),

@@ -114,6 +114,15 @@ def test_oracle_matches_reference_binaries_fresh(seed):
     assert np.array_equal(O.sssp_serial(g, src), O.ref_run("sssp_edge_data", g, src))
     assert np.array_equal(O.cc_omp_push(g)[0], O.ref_run("cc_push", g))
     assert np.array_equal(O.mis_greedy(g), O.ref_run("mis_vertex", g))
+    # the reference's Determ forms (state double-buffered per round or pass)
+    # reach the same fixed points as the NonDeterm ones
+    determ = (("bfs_vtopo_pull_determ", O.bfs_serial(g, src)), ("bfs_vdata_push_determ", O.bfs_serial(g, src)),
+              ("sssp_edata_determ", O.sssp_serial(g, src)), ("sssp_vtopo_pull_determ", O.sssp_serial(g, src)),
+              ("sssp_vdata_push_determ", O.sssp_serial(g, src)),
+              ("cc_vtopo_pull_determ", O.cc_omp_push(g)[0]), ("cc_vdata_push_determ", O.cc_omp_push(g)[0]))
+    for v, want in determ:
+        if O.ref_available(v):
+            assert np.array_equal(want, O.ref_run(v, g, src)), v
     # the reference's edge worklist Determ forms (statuses double-buffered per
     # pass) reach the same greedy set as the vertex form
     for v in ("mis_edata_copy", "mis_edata_merge"):
