@@ -119,6 +119,16 @@ int main(int argc, char* argv[])
   CPUtc_vertex(count, g.nodes, g.nindex, g.nlist, ref_threads(argc, argv));
   long long c = count;
   dump(argv[2], &c, sizeof(c));
+#elif REF_KIND == 16 /* BFS edge: CPUbfs_edge(src, g, dist, sp) */
+  int* sp = make_sp(g);
+  data_type* out = new data_type [g.nodes];
+  CPUbfs_edge(src, g, out, sp);
+  dump(argv[2], out, sizeof(data_type) * g.nodes);
+#elif REF_KIND == 17 /* CC edge: CPUcc_edge(g, label, sp) */
+  int* sp = make_sp(g);
+  data_type* out = new data_type [g.nodes];
+  CPUcc_edge(g, out, sp);
+  dump(argv[2], out, sizeof(data_type) * g.nodes);
 #else
 #error "unknown REF_KIND"
 #endif

@@ -119,7 +119,12 @@ def test_oracle_matches_reference_binaries_fresh(seed):
     determ = (("bfs_vtopo_pull_determ", O.bfs_serial(g, src)), ("bfs_vdata_push_determ", O.bfs_serial(g, src)),
               ("sssp_edata_determ", O.sssp_serial(g, src)), ("sssp_vtopo_pull_determ", O.sssp_serial(g, src)),
               ("sssp_vdata_push_determ", O.sssp_serial(g, src)),
-              ("cc_vtopo_pull_determ", O.cc_omp_push(g)[0]), ("cc_vdata_push_determ", O.cc_omp_push(g)[0]))
+              ("cc_vtopo_pull_determ", O.cc_omp_push(g)[0]), ("cc_vdata_push_determ", O.cc_omp_push(g)[0]),
+              # edge-based forms (edge -> row map built by the dump driver)
+              ("bfs_edata", O.bfs_serial(g, src)), ("bfs_edata_rw", O.bfs_serial(g, src)),
+              ("bfs_edata_determ", O.bfs_serial(g, src)), ("bfs_etopo_determ", O.bfs_serial(g, src)),
+              ("cc_edata", O.cc_omp_push(g)[0]), ("cc_edata_rw", O.cc_omp_push(g)[0]),
+              ("cc_edata_determ", O.cc_omp_push(g)[0]), ("cc_etopo_determ", O.cc_omp_push(g)[0]))
     for v, want in determ:
         if O.ref_available(v):
             assert np.array_equal(want, O.ref_run(v, g, src)), v
