@@ -291,6 +291,7 @@ REF_OUT = {  # name -> (dtype, is_scalar)
     "bfs_edata": (np.int32, False), "bfs_edata_rw": (np.int32, False), "bfs_edata_determ": (np.int32, False),
     "bfs_etopo_determ": (np.int32, False), "cc_edata": (np.int32, False), "cc_edata_rw": (np.int32, False),
     "cc_edata_determ": (np.int32, False), "cc_etopo_determ": (np.int32, False),
+    "pr_push": (np.float32, False), "pr_push_clausered": (np.float32, False), "tc_vertex_atomic": (np.int64, True), "tc_edge_critical": (np.int64, True),
     "pr_pull": (np.float32, False), "tc_vertex": (np.int64, True), "tc_edge": (np.int64, True),
     "cpp_bfs": (np.int32, False), "cpp_sssp": (np.int32, False), "cpp_cc": (np.int32, False),
     "cpp_mis": (np.uint8, False), "cpp_pr": (np.float32, False), "cpp_tc": (np.int64, True),

@@ -134,6 +134,24 @@ def test_oracle_matches_reference_binaries_fresh(seed):
         if O.ref_available(v):
             assert np.array_equal(O.mis_greedy(g), O.ref_run(v, g)), v
     assert O.tc_serial(g) == O.ref_run("tc_edge", g)
+    for v in ("tc_vertex_atomic", "tc_edge_critical"):
+        if O.ref_available(v):
+            assert O.tc_serial(g) == O.ref_run(v, g), v
+    if O.ref_available("pr_push"):
+        # PR_OMP_Push_..._AtomicRed: push order changes the float sums and may
+        # change the stopping iteration; both runs stop within L1 < 1e-4 of a
+        # step, so their vectors (L1 norm about 1) agree far better than a
+        # wrong formula would (a missing damping term moves them by more than
+        # 0.1)
+        sc, _, _ = O.pr_omp(g, 1e-4, 100)
+        assert float(np.sum(np.abs(O.ref_run("pr_push", g) - sc))) < 2e-3
+    if O.ref_available("pr_push_clausered"):
+        # PR_OMP_Push_..._ClauseRed adds into incoming_total[nei] from several
+        # threads without synchronisation (codes/omp/pr-omp/PR_OMP_Push_
+        # FloatType_DefaultSched_ClauseRed.cpp:60), so only its one-thread run
+        # is a fixed reference: it equals the pull oracle's result exactly
+        sc, _, _ = O.pr_omp(g, 1e-4, 100)
+        assert np.array_equal(O.ref_run("pr_push_clausered", g, threads=1), sc)
 
 
 # ------------------------------------- C++-threads restatement ----
