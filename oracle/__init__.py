@@ -295,6 +295,11 @@ REF_OUT = {  # name -> (dtype, is_scalar)
     "pr_pull": (np.float32, False), "tc_vertex": (np.int64, True), "tc_edge": (np.int64, True),
     "cpp_bfs": (np.int32, False), "cpp_sssp": (np.int32, False), "cpp_cc": (np.int32, False),
     "cpp_mis": (np.uint8, False), "cpp_pr": (np.float32, False), "cpp_tc": (np.int64, True),
+    "cpp_bfs_vdata_push_determ": (np.int32, False), "cpp_bfs_vtopo_pull_determ": (np.int32, False),
+    "cpp_sssp_vdata_push_determ": (np.int32, False), "cpp_cc_vtopo_pull_determ": (np.int32, False),
+    "cpp_cc_vdata_push_determ": (np.int32, False), "cpp_mis_vtopo_pull_copydeterm": (np.uint8, False),
+    "cpp_mis_vdata_pull_copydeterm": (np.uint8, False), "cpp_pr_push_atomicadd": (np.float32, False),
+    "cpp_tc_vertex_critical": (np.int64, True),
 }
 
 

@@ -229,6 +229,36 @@ def test_cpp_restatement_matches_reference_binaries_fresh(seed):
         assert O.cpp_tc(g, threads)[0] == O.ref_run("cpp_tc", g, 0, threads)
 
 
+@pytest.mark.skipif(not reference_present(), reason="reference tree not mounted")
+def test_cpp_reference_variant_families_fresh():
+    """More codes/cpp variants (data-driven and pull forms, Determ and
+    CopyDeterm, PR push with atomic adds, TC with a critical section) reach
+    the C++-threads restatement's results at 1 and 6 threads."""
+    names = ("cpp_bfs_vdata_push_determ", "cpp_bfs_vtopo_pull_determ", "cpp_sssp_vdata_push_determ",
+             "cpp_cc_vtopo_pull_determ", "cpp_cc_vdata_push_determ", "cpp_mis_vtopo_pull_copydeterm",
+             "cpp_mis_vdata_pull_copydeterm", "cpp_pr_push_atomicadd", "cpp_tc_vertex_critical")
+    if not all(O.ref_available(v) for v in names):
+        pytest.skip("oracle/_ref not built")
+    g = G.chung_lu(3000, 12, 2.2, seed=33).with_weights(1, 255, seed=33)
+    src = g.max_degree_vertex()
+    for threads in (1, 6):
+        bfs = O.cpp_bfs(g, src, threads)[0]
+        assert np.array_equal(bfs, O.ref_run("cpp_bfs_vdata_push_determ", g, src, threads))
+        assert np.array_equal(bfs, O.ref_run("cpp_bfs_vtopo_pull_determ", g, src, threads))
+        assert np.array_equal(O.cpp_sssp(g, src, threads)[0], O.ref_run("cpp_sssp_vdata_push_determ", g, src, threads))
+        cc = O.cpp_cc(g, threads)[0]
+        assert np.array_equal(cc, O.ref_run("cpp_cc_vtopo_pull_determ", g, 0, threads))
+        assert np.array_equal(cc, O.ref_run("cpp_cc_vdata_push_determ", g, 0, threads))
+        mis = O.cpp_mis(g, threads)[0]
+        assert np.array_equal(mis, O.ref_run("cpp_mis_vtopo_pull_copydeterm", g, 0, threads))
+        assert np.array_equal(mis, O.ref_run("cpp_mis_vdata_pull_copydeterm", g, 0, threads))
+        # push adds floats in thread order: compare the vectors (L1 norm
+        # about 1) within what a step below the 1e-4 stopping residual allows
+        pr = O.cpp_pr(g, 1e-4, 100, threads)[0]
+        assert float(np.sum(np.abs(O.ref_run("cpp_pr_push_atomicadd", g, 0, threads) - pr))) < 2e-3
+        assert O.cpp_tc(g, threads)[0] == O.ref_run("cpp_tc_vertex_critical", g, 0, threads)
+
+
 # --------------------------------------------------------- textbook ----
 @pytest.mark.parametrize("make", [lambda: G.erdos_renyi(16384, 8, seed=1), lambda: G.rmat(13, 16, seed=4),
                                   lambda: G.grid2d(50, 0.05, seed=2)])
